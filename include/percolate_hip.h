@@ -1,0 +1,128 @@
+/*
+ * percolate_hip.h -- C ABI of libpercolate_hip.so, the MI355X (gfx950)
+ * implementation of pypercolate's Newman-Ziff bond-percolation hot path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: PC_OK (0) or a negative
+ *     PC_E* code; pc_last_error() then holds a thread-local message.  No
+ *     exception or abort crosses the ABI.
+ *   - Pointers named d_* are device (HBM) pointers owned by the caller; the
+ *     library allocates nothing behind them.  `stream` is a hipStream_t
+ *     (NULL = the default stream); work is enqueued asynchronously on it.
+ *   - Graph: perc nodes are 0..num_nodes-1, edges are int32 pairs
+ *     d_edges[2*e], d_edges[2*e+1].  d_node_sides[i] is 0, 1 (touches the
+ *     first spanning side) or 2 (touches the second), or d_node_sides is NULL
+ *     when spanning clusters are not detected.  span_on_first_merge = 1 when
+ *     the auxiliary structure alone already joins both sides (the reference
+ *     then reports a spanning cluster from the first merge on).
+ *   - Steps: step s = 0..num_edges is the state after s bonds.  perm[s-1] is
+ *     the edge index added at step s.
+ *   - Merge records: one per step s >= 1, at index s-1.  0 = the edge joined
+ *     two nodes of one cluster; otherwise the sizes (a, b) of the two
+ *     clusters it merged, packed a | b << 16 in a uint32 when
+ *     num_nodes <= 65536 and a | b << 32 in a uint64 above
+ *     (pc_record_bytes()).
+ *   - Spanning: span_step[r] is the first step with a spanning cluster, or
+ *     0xFFFFFFFF if none.
+ */
+#ifndef PERCOLATE_HIP_H
+#define PERCOLATE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PC_OK 0
+#define PC_EINVAL (-1)    /* bad argument (maps to ValueError)              */
+#define PC_ENOMEM (-2)    /* workspace too small (maps to MemoryError)      */
+#define PC_EHIP (-3)      /* HIP runtime error (maps to RuntimeError)       */
+#define PC_EOVERFLOW (-4) /* size outside the supported range (ValueError)  */
+
+#define PC_NO_SPAN 0xFFFFFFFFu
+#define PC_CHUNK 64       /* steps per chunk-state entry                    */
+#define PC_STATE_WORDS 8  /* uint64 words per chunk-state entry             */
+#define PC_COLUMNS 7      /* max, spanning, moments 0..4                    */
+
+/* ABI revision of this header. */
+int pc_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *pc_last_error(void);
+
+/* Bytes per merge record for a graph of num_nodes nodes (4 or 8). */
+int pc_record_bytes(uint32_t num_nodes);
+
+/*
+ * Device permutations (performance mode).  Fills d_perm[r * perm_stride + i]
+ * for r < num_runs, i < num_edges with a pseudo-random permutation of
+ * 0..num_edges-1 for run first_run + r: a cycle-walking Feistel network whose
+ * eight round functions are tables drawn from Philox4x32-10 keyed by seed.
+ * Replaces the host-side numpy RandomState(seed).permutation(num_edges) of
+ * percolate/hpc.py:195,206 when bit-exactness with numpy is not required.
+ */
+int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
+                      uint32_t num_edges, int32_t *d_perm, int64_t perm_stride,
+                      void *stream);
+
+/* Device workspace bytes pc_sweep needs (0 when the state fits in LDS). */
+int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
+                            uint64_t *bytes);
+
+/*
+ * The Newman-Ziff sweep of num_runs independent runs, one per wavefront:
+ * replaces the per-edge loop of percolate/hpc.py:249-307 (and its twin
+ * percolate/percolate.py:298-356).  Reads d_perm[r * perm_stride + n],
+ * writes merge records d_records[r * rec_stride + n] (n < num_edges) and
+ * d_span_step[r].  Union-find state lives in LDS (num_nodes <= 16384) or in
+ * d_workspace (pc_sweep_workspace_size bytes).
+ */
+int pc_sweep(uint32_t num_nodes, uint32_t num_edges, const int32_t *d_edges,
+             const uint8_t *d_node_sides, int32_t span_on_first_merge,
+             const int32_t *d_perm, int64_t perm_stride, uint32_t num_runs,
+             void *d_records, int64_t rec_stride, uint32_t *d_span_step,
+             void *d_workspace, uint64_t workspace_bytes, void *stream);
+
+/*
+ * Per-run state at the start of every chunk of PC_CHUNK steps, from the
+ * merge records: d_chunk_state[(r * C + c) * 8 + w], C = ceil(num_edges/64),
+ * words {max cluster size, merges, S2, S3 lo, S3 hi, S4 lo, S4 hi, 0} after
+ * c * 64 bonds, where Sk is the sum over all clusters of size**k.
+ */
+int pc_chunk_scan(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
+                  const void *d_records, int64_t rec_stride,
+                  uint64_t *d_chunk_state, void *stream);
+
+/*
+ * Microcanonical reduction over runs, exact: for every step s = 0..num_edges
+ * and column k (max_cluster_size, has_spanning_cluster, moments[0..4]) the
+ * sum over runs of x (d_sum1[(k * S + s) * 2 + limb], uint128) and of x**2
+ * (d_sum2[(k * S + s) * 3 + limb], uint192), S = num_edges + 1.  Replaces the
+ * per-step gathers of percolate/percolate.py:862-891 feeding
+ * _microcanonical_average_* (percolate.py:575-705).  Moments are exact
+ * integers (their low 64 bits are the reference's uint64 values).
+ */
+int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
+                           uint32_t num_runs, const void *d_records,
+                           int64_t rec_stride, const uint32_t *d_span_step,
+                           const uint64_t *d_chunk_state, uint64_t *d_sum1,
+                           uint64_t *d_sum2, void *stream);
+
+/*
+ * Per-run traces, the arrays of percolate/hpc.py:310-404
+ * (bond_microcanonical_statistics): d_max_size[r * S + s] (uint32),
+ * d_spanning[r * S + s] (uint8) and d_moments[(r * S + s) * 5 + k] (uint64,
+ * the reference's mod 2**64 values), S = num_edges + 1.
+ */
+int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
+                     const void *d_records, int64_t rec_stride,
+                     const uint32_t *d_span_step, const uint64_t *d_chunk_state,
+                     uint32_t *d_max_size, uint8_t *d_spanning,
+                     uint64_t *d_moments, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PERCOLATE_HIP_H */

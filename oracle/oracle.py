@@ -1,0 +1,445 @@
+"""TEST INFRASTRUCTURE ONLY: CPU oracle of the Newman-Ziff hot path.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module, and only as the checker; the product package
+(pypercolate_amd) never imports, calls or links it.
+
+Each function restates the reference (andsor/pypercolate 0.4.6) step for
+step, citing the lines it follows:
+
+* ``percolation_graph_arrays``  percolate/percolate.py:31-100 (networkx 3 API)
+* ``bond_runs``                 percolate/hpc.py:194-307 via nz_oracle.c
+* ``microcanonical_sums``       exact sums over runs (nz_oracle.c)
+* ``binomial_pmf``              percolate/percolate.py:1067-1109
+* ``canonical_statistics``      percolate/hpc.py:443-515
+* ``initialize_canonical_averages``  percolate/hpc.py:561-635
+* ``online_variance``           simoa.stats.online_variance (Chan et al.), the
+                                third-party merge behind hpc.py:677-692
+* ``bond_reduce``               percolate/hpc.py:638-702
+* ``finalize_canonical_averages``  percolate/hpc.py:752-834
+* ``microcanonical_step_stats`` percolate/percolate.py:450-705, 862-891
+* ``microcanonical_arrays``     percolate/percolate.py:968-1064
+* ``canonical_averages``        percolate/percolate.py:1112-1223
+* ``philox4x32_10`` and ``feistel_permutation``: the device-permutation
+  algorithm of this package (no reference counterpart), restated in Python.
+
+Pinning (tests/test_oracle_*.py): the sweep against the reference's own
+known-answer tests, the tutorial traces, brute-force connected components
+and closed-form end states; the binomial weights against scipy.stats.binom;
+the Chan merge against numpy mean/variance; finalize against scipy's t CDF;
+Philox against the Random123 known-answer vectors.  ``online_variance`` is
+pinned to tolerance only: simoa's exact floating-point operation order is
+parity unpinned (simoa is absent here, and unpinned in requirements.txt).
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import scipy.stats
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libnzoracle.so')
+
+_lib = None
+
+
+def build():
+    """Compile nz_oracle.c into oracle/libnzoracle.so."""
+    subprocess.check_call(['make', '-s', '-C', HERE])
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        lib.nzo_bond_runs.restype = ctypes.c_int
+        lib.nzo_bond_runs.argtypes = [i64, i64, i64, vp, ctypes.c_int, i64,
+                                      vp, i64, vp, vp, vp, vp, vp, vp, vp,
+                                      ctypes.c_int]
+        lib.nzo_microcanonical_sums.restype = None
+        lib.nzo_microcanonical_sums.argtypes = [i64, i64, vp, vp, vp, vp, vp]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Graph preparation: percolate/percolate.py:31-100 with networkx 3 calls
+# ---------------------------------------------------------------------------
+def percolation_graph_arrays(graph, spanning_cluster=True):
+    """Integer arrays of percolation_graph(graph, spanning_cluster).
+
+    Perc nodes are numbered in perc_graph.nodes() order, edges in
+    perc_graph.edges() order; auxiliary nodes follow the perc nodes.
+    """
+    ret = {'spanning': bool(spanning_cluster)}
+    if spanning_cluster:
+        aux_attr = {n: d['span'] for n, d in graph.nodes(data=True) if 'span' in d}
+        if not aux_attr:
+            raise ValueError('no auxiliary nodes')
+        sides = list(set(aux_attr.values()))
+        if len(sides) != 2:
+            raise ValueError('auxiliary nodes of less or more than 2 sides')
+        perc_graph = graph.subgraph(
+            [n for n in graph.nodes() if 'span' not in graph.nodes[n]])
+    else:
+        perc_graph = graph
+        aux_attr = {}
+        sides = []
+    nodes = list(perc_graph.nodes())
+    index = {n: i for i, n in enumerate(nodes)}
+    aux_nodes = list(aux_attr)
+    for j, n in enumerate(aux_nodes):
+        index[n] = len(nodes) + j
+    edge_list = list(perc_graph.edges())
+    ret['num_nodes'] = len(nodes)
+    ret['num_edges'] = len(edge_list)
+    ret['nodes'] = nodes
+    ret['edge_list'] = edge_list
+    ret['edges'] = np.array([(index[u], index[v]) for u, v in edge_list],
+                            dtype=np.int64).reshape(-1, 2)
+    ret['aux_side'] = np.array([sides.index(aux_attr[n]) for n in aux_nodes],
+                               dtype=np.int32)
+    aux_edges = []
+    aux_edge_side = []
+    if spanning_cluster:
+        for (u, v), side in _edge_span_attributes(graph).items():
+            aux_edges.append((index[u], index[v]))
+            aux_edge_side.append(sides.index(side))
+    ret['aux_edges'] = np.array(aux_edges, dtype=np.int64).reshape(-1, 2)
+    ret['aux_edge_side'] = np.array(aux_edge_side, dtype=np.int32)
+    return ret
+
+
+def _edge_span_attributes(graph):
+    return {(u, v): d['span'] for u, v, d in graph.edges(data=True) if 'span' in d}
+
+
+# ---------------------------------------------------------------------------
+# Sweep (nz_oracle.c)
+# ---------------------------------------------------------------------------
+def bond_runs(g, perms, threads=None):
+    """Traces of runs with the given permutations (R, M).
+
+    Returns dict edge (R,S) u32, has_spanning_cluster (R,S) bool,
+    max_cluster_size (R,S) u32, moments (R,S,5) u64; S = M + 1.
+    """
+    lib = _load()
+    M = g['num_edges']
+    perms = np.ascontiguousarray(perms, dtype=np.int64).reshape(-1, M) if M else \
+        np.zeros((np.asarray(perms).shape[0], 0), dtype=np.int64)
+    R = perms.shape[0]
+    S = M + 1
+    edge = np.zeros((R, S), dtype=np.uint32)
+    span = np.zeros((R, S), dtype=np.uint8)
+    mx = np.zeros((R, S), dtype=np.uint32)
+    mom = np.zeros((R, S, 5), dtype=np.uint64)
+    edges = np.ascontiguousarray(g['edges'], dtype=np.int64)
+    aux_side = np.ascontiguousarray(g['aux_side'], dtype=np.int32)
+    aux_edges = np.ascontiguousarray(g['aux_edges'], dtype=np.int64)
+    aux_edge_side = np.ascontiguousarray(g['aux_edge_side'], dtype=np.int32)
+    rc = lib.nzo_bond_runs(
+        R, g['num_nodes'], M, _p(edges), int(g['spanning']), aux_side.size,
+        _p(aux_side), aux_edge_side.size, _p(aux_edges), _p(aux_edge_side),
+        _p(perms), _p(edge), _p(span), _p(mx), _p(mom),
+        int(threads or os.cpu_count() or 1))
+    if rc:
+        raise ValueError('oracle: an auxiliary side has no nodes')
+    return {'edge': edge, 'has_spanning_cluster': span.astype(bool),
+            'max_cluster_size': mx, 'moments': mom}
+
+
+def microcanonical_sums(traces):
+    """Exact sums over runs per step: (sum1, sum2) object arrays (7, S) of
+    Python ints, columns max, spanning, moments 0..4."""
+    lib = _load()
+    span = np.ascontiguousarray(traces['has_spanning_cluster'], dtype=np.uint8)
+    mx = np.ascontiguousarray(traces['max_cluster_size'], dtype=np.uint32)
+    mom = np.ascontiguousarray(traces['moments'], dtype=np.uint64)
+    R, S = mx.shape
+    s1 = np.zeros((7, S, 2), dtype=np.uint64)
+    s2 = np.zeros((7, S, 3), dtype=np.uint64)
+    lib.nzo_microcanonical_sums(R, S, _p(span), _p(mx), _p(mom), _p(s1), _p(s2))
+    return limbs_to_int(s1), limbs_to_int(s2)
+
+
+def limbs_to_int(limbs):
+    limbs = np.asarray(limbs).view(np.uint64)
+    out = np.zeros(limbs.shape[:-1], dtype=object)
+    for j in range(limbs.shape[-1] - 1, -1, -1):
+        out = out * (1 << 64) + limbs[..., j].astype(object)
+    return out
+
+
+def randomstate_permutations(seeds, num_edges):
+    """numpy RandomState(seed).permutation(M) per seed (hpc.py:195,206)."""
+    return np.stack([np.random.RandomState(seed=s).permutation(num_edges)
+                     for s in seeds]).astype(np.int64) if len(seeds) else \
+        np.zeros((0, num_edges), dtype=np.int64)
+
+
+# ---------------------------------------------------------------------------
+# Canonical ensemble: percolate/percolate.py:1067-1109 and hpc.py:443-834
+# ---------------------------------------------------------------------------
+def binomial_pmf(n, p):
+    """percolate/percolate.py:1067-1109: ratio recursion outward from
+    round(p n), then normalised by the sum."""
+    n = int(n)
+    ret = np.empty(n + 1)
+    nmax = int(np.round(p * n))
+    ret[nmax] = 1.0
+    old = np.seterr(under='ignore', divide='ignore', invalid='ignore')
+    for i in range(nmax + 1, n + 1):
+        ret[i] = ret[i - 1] * (n - i + 1.0) / i * p / (1.0 - p)
+    for i in range(nmax - 1, -1, -1):
+        ret[i] = ret[i + 1] * (i + 1.0) / (n - i) * (1.0 - p) / p
+    np.seterr(**old)
+    return ret / ret.sum()
+
+
+def canonical_statistics(trace, convolution_factors):
+    """percolate/hpc.py:443-515 for one run and one p: dict of float64."""
+    f = np.asarray(convolution_factors, dtype=np.float64)
+    ret = {}
+    if 'has_spanning_cluster' in trace:
+        ret['percolation_probability'] = np.sum(f * trace['has_spanning_cluster'])
+    ret['max_cluster_size'] = np.sum(f * trace['max_cluster_size'])
+    ret['moments'] = np.sum(f[:, np.newaxis] * trace['moments'], axis=0)
+    return ret
+
+
+def initialize_canonical_averages(stats_list):
+    """percolate/hpc.py:561-635 over a list of canonical_statistics dicts
+    (one per p): n = 1, mean = value, m2 = 0."""
+    out = {'number_of_runs': np.ones(len(stats_list), dtype=np.uint32)}
+    for key in stats_list[0]:
+        vals = np.array([s[key] for s in stats_list], dtype=np.float64)
+        out[key + '_mean'] = vals
+        out[key + '_m2'] = np.zeros_like(vals)
+    return out
+
+
+def online_variance(a, b):
+    """Pairwise (n, mean, M2) merge of Chan, Golub & LeVeque (the algorithm
+    simoa.stats.online_variance implements)."""
+    n_a, mean_a, m2_a = a
+    n_b, mean_b, m2_b = b
+    n_a = np.asarray(n_a, dtype=np.float64)
+    n_b = np.asarray(n_b, dtype=np.float64)
+    n = n_a + n_b
+    delta = np.asarray(mean_b) - np.asarray(mean_a)
+    mean = np.asarray(mean_a) + delta * n_b / n
+    m2 = np.asarray(m2_a) + np.asarray(m2_b) + delta ** 2 * n_a * n_b / n
+    return n, mean, m2
+
+
+def bond_reduce(a, b):
+    """percolate/hpc.py:638-702 on dicts of arrays."""
+    out = {'number_of_runs': a['number_of_runs'] + b['number_of_runs']}
+    for key in a:
+        if not key.endswith('_mean'):
+            continue
+        base = key[:-5]
+        na = a['number_of_runs'].astype(np.float64)
+        nb = b['number_of_runs'].astype(np.float64)
+        if a[key].ndim == 2:
+            na, nb = na[:, None], nb[:, None]
+        _, mean, m2 = online_variance((na, a[key], a[base + '_m2']),
+                                      (nb, b[key], b[base + '_m2']))
+        out[key] = mean
+        out[base + '_m2'] = m2
+    return out
+
+
+def finalize_canonical_averages(number_of_nodes, ps, avg, alpha):
+    """percolate/hpc.py:752-834 on dicts of arrays."""
+    n = avg['number_of_runs'].astype(np.float64)
+    sqrt_n = np.sqrt(n)
+    out = {'number_of_runs': avg['number_of_runs'], 'p': np.asarray(ps),
+           'alpha': alpha}
+
+    def transform(key, final, normalize):
+        mean = np.array(avg[key + '_mean'], dtype=np.float64)
+        m2 = avg[key + '_m2']
+        nn = n[:, None] if m2.ndim == 2 else n
+        std = np.sqrt(m2 / (nn - 1))
+        if normalize:
+            mean = mean / number_of_nodes
+            std = std / number_of_nodes
+        sq = sqrt_n[:, None] if m2.ndim == 2 else sqrt_n
+        lo, hi = scipy.stats.t.interval(1 - alpha, df=nn - 1, loc=mean,
+                                        scale=std / sq)
+        out[final + '_mean'] = mean
+        out[final + '_std'] = std
+        out[final + '_ci'] = np.stack([lo, hi], axis=-1)
+
+    if 'percolation_probability_mean' in avg:
+        transform('percolation_probability', 'percolation_probability', False)
+    transform('max_cluster_size', 'percolation_strength', True)
+    transform('moments', 'moments', True)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Microcanonical averages: percolate/percolate.py:450-705, 862-1064
+# ---------------------------------------------------------------------------
+def microcanonical_step_stats(max_sizes, moments, spanning, alpha):
+    """One step over R runs: the dict the reference's per-step helpers build
+    (percolate.py:557-570, 613-635, 681-705).  Inputs are float arrays."""
+    ret = {}
+    runs = max_sizes.size
+    sqrt_n = np.sqrt(runs)
+    m = max_sizes.mean()
+    ret['max_cluster_size'] = m
+    sd = max_sizes.std(ddof=1)
+    if sd:
+        ret['max_cluster_size_ci'] = np.array(scipy.stats.t.interval(
+            1 - alpha, df=runs - 1, loc=m, scale=sd / sqrt_n))
+    else:
+        ret['max_cluster_size_ci'] = m * np.ones(2)
+    mm = moments.mean(axis=0)
+    ret['moments'] = mm
+    msd = moments.std(axis=0, ddof=1)
+    ci = np.empty((5, 2))
+    for k in range(5):
+        if msd[k]:
+            ci[k] = scipy.stats.t.interval(1 - alpha, df=runs - 1, loc=mm[k],
+                                           scale=msd[k] / sqrt_n)
+        else:
+            ci[k] = mm[k] * np.ones(2)
+    ret['moments_ci'] = ci
+    if spanning is not None:
+        k = spanning.sum(dtype=np.float64)
+        ret['spanning_cluster'] = (k + 1) / (runs + 2)
+        ret['spanning_cluster_ci'] = scipy.stats.beta.ppf(
+            [alpha / 2, 1 - alpha / 2], k + 1, runs - k + 1)
+    return ret
+
+
+def microcanonical_arrays(traces, num_nodes, alpha):
+    """percolate.py:708-896 + 968-1064 from per-run traces: stacked arrays,
+    all but spanning divided by N."""
+    mx = traces['max_cluster_size'].astype(np.float64)
+    mom = traces['moments'].astype(np.float64)
+    span = traces.get('has_spanning_cluster')
+    R, S = mx.shape
+    out = {'max_cluster_size': np.empty(S), 'max_cluster_size_ci': np.empty((S, 2)),
+           'moments': np.empty((5, S)), 'moments_ci': np.empty((5, S, 2))}
+    if span is not None:
+        out['spanning_cluster'] = np.empty(S)
+        out['spanning_cluster_ci'] = np.empty((S, 2))
+    for s in range(S):
+        st = microcanonical_step_stats(
+            mx[:, s], mom[:, s, :], None if span is None else span[:, s], alpha)
+        out['max_cluster_size'][s] = st['max_cluster_size']
+        out['max_cluster_size_ci'][s] = st['max_cluster_size_ci']
+        out['moments'][:, s] = st['moments']
+        out['moments_ci'][:, s] = st['moments_ci']
+        if span is not None:
+            out['spanning_cluster'][s] = st['spanning_cluster']
+            out['spanning_cluster_ci'][s] = st['spanning_cluster_ci']
+    for key in list(out):
+        if 'spanning_cluster' in key:
+            continue
+        out[key] = out[key] / num_nodes
+    out['N'] = num_nodes
+    out['M'] = S - 1
+    return out
+
+
+def canonical_averages(ps, arrays):
+    """percolate/percolate.py:1112-1223."""
+    ps = np.asarray(ps)
+    M = arrays['M']
+    ret = {'ps': ps, 'N': arrays['N'], 'M': M}
+    ret['max_cluster_size'] = np.empty(ps.size)
+    ret['max_cluster_size_ci'] = np.empty((ps.size, 2))
+    if 'spanning_cluster' in arrays:
+        ret['spanning_cluster'] = np.empty(ps.size)
+        ret['spanning_cluster_ci'] = np.empty((ps.size, 2))
+    ret['moments'] = np.empty((5, ps.size))
+    ret['moments_ci'] = np.empty((5, ps.size, 2))
+    for i, p in enumerate(ps):
+        b = binomial_pmf(M, p)
+        for key, value in arrays.items():
+            if len(key) <= 1:
+                continue
+            if key in ('max_cluster_size', 'spanning_cluster'):
+                ret[key][i] = np.sum(b * value)
+            elif key in ('max_cluster_size_ci', 'spanning_cluster_ci'):
+                ret[key][i] = np.sum(np.tile(b, (2, 1)).T * value, axis=0)
+            elif key == 'moments':
+                ret[key][:, i] = np.sum(np.tile(b, (5, 1)) * value, axis=1)
+            elif key == 'moments_ci':
+                ret[key][:, i] = np.sum(
+                    np.rollaxis(np.tile(b, (5, 2, 1)), 2, 1) * value, axis=1)
+    return ret
+
+
+# ---------------------------------------------------------------------------
+# Device-permutation algorithm (this package's own; no reference counterpart)
+# ---------------------------------------------------------------------------
+PHILOX_M0, PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
+PHILOX_W0, PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
+MASK32 = 0xFFFFFFFF
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32 with 10 rounds (Salmon et al., SC'11) on Python ints."""
+    c0, c1, c2, c3 = [int(x) & MASK32 for x in ctr]
+    k0, k1 = [int(x) & MASK32 for x in key]
+    for i in range(10):
+        if i:
+            k0 = (k0 + PHILOX_W0) & MASK32
+            k1 = (k1 + PHILOX_W1) & MASK32
+        p0 = PHILOX_M0 * c0
+        p1 = PHILOX_M1 * c2
+        c0, c1, c2, c3 = ((p1 >> 32) ^ c1 ^ k0, p1 & MASK32,
+                          (p0 >> 32) ^ c3 ^ k1, p0 & MASK32)
+    return c0, c1, c2, c3
+
+
+PERM_ROUNDS = 8
+PERM_TAG = 0x7065726d
+
+
+def feistel_permutation(seed, run, num_edges):
+    """The permutation pc_permute_philox draws for one run (Python ints)."""
+    M = int(num_edges)
+    if M == 0:
+        return np.zeros(0, dtype=np.int64)
+    bits = 1
+    while (1 << bits) < M:
+        bits += 1
+    h = (bits + 1) // 2
+    mask = (1 << h) - 1
+    entries = PERM_ROUNDS << h
+    table = np.zeros(entries, dtype=np.int64)
+    for g in range((entries + 7) // 8):
+        words = philox4x32_10((g, run & MASK32, run >> 32, PERM_TAG),
+                              (seed & MASK32, (seed >> 32) & MASK32))
+        for j in range(8):
+            e = g * 8 + j
+            if e < entries:
+                table[e] = (words[j >> 1] >> (16 * (j & 1))) & mask
+    out = np.empty(M, dtype=np.int64)
+    for i in range(M):
+        y = i
+        while True:
+            L, R = y >> h, y & mask
+            for j in range(PERM_ROUNDS):
+                L, R = R, L ^ int(table[(j << h) | R])
+            y = (L << h) | R
+            if y < M:
+                break
+        out[i] = y
+    return out

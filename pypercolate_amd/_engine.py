@@ -1,0 +1,167 @@
+"""Device pipeline of the Newman-Ziff hot path on one GPU.
+
+``BondEngine`` holds one graph in HBM and runs batches of runs through
+libpercolate_hip: permutations (uploaded from numpy RandomState for bit
+parity with the reference, or drawn on the device), the union-find sweep,
+the per-run chunk states, and then either the exact microcanonical sums over
+runs or per-run traces.  All buffers are torch tensors on the current
+device; kernels run on torch's current stream.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+
+class SweepResult(object):
+    """Device outputs of one batch of runs."""
+
+    def __init__(self, engine, num_runs, records, span_step, perm=None):
+        self.engine = engine
+        self.num_runs = num_runs
+        self.records = records        # (R, M) int32/int64 tensor (u32/u64 bits)
+        self.span_step = span_step    # (R,) int32 tensor (u32 bits)
+        self.perm = perm              # (R, M) int32 tensor or None
+        self._chunk_state = None
+
+    @property
+    def chunk_state(self):
+        if self._chunk_state is None:
+            self._chunk_state = self.engine.chunk_scan(self)
+        return self._chunk_state
+
+
+class BondEngine(object):
+    """One percolation graph resident on the GPU."""
+
+    def __init__(self, graph):
+        self.torch = torch = nat.torch_cuda()
+        self.lib = nat.load_library()
+        self.graph = graph
+        self.N = graph.num_nodes
+        self.M = graph.num_edges
+        self.S = self.M + 1
+        dev = torch.device('cuda', torch.cuda.current_device())
+        self.device = dev
+        self.d_edges = torch.from_numpy(graph.edges.reshape(-1).copy()).to(dev)
+        self.d_sides = (
+            None if graph.node_sides is None
+            else torch.from_numpy(graph.node_sides.copy()).to(dev)
+        )
+        self.span_first = 1 if graph.span_on_first_merge else 0
+        self.rec_bytes = self.lib.pc_record_bytes(self.N)
+        self.rec_torch = torch.int32 if self.rec_bytes == 4 else torch.int64
+        self.rec_numpy = nat.record_dtype(self.N)
+
+    # -- permutations -----------------------------------------------------
+    def upload_permutations(self, perms):
+        """(R, M) host permutations -> device int32 tensor."""
+        perms = np.asarray(perms)
+        if perms.size and (perms.min() < 0 or perms.max() >= self.M):
+            raise ValueError('permutation entries must be edge indices < M')
+        perms = np.ascontiguousarray(perms, dtype=np.int32).reshape(-1, self.M)
+        return self.torch.from_numpy(perms).to(self.device)
+
+    def device_permutations(self, seed, num_runs, first_run=0, out=None):
+        """Philox-keyed device permutations for runs first_run..+num_runs."""
+        torch = self.torch
+        if out is None:
+            out = torch.empty((num_runs, self.M), dtype=torch.int32,
+                              device=self.device)
+        st = nat.stream_handle(torch)
+        nat.check(self.lib.pc_permute_philox(
+            int(seed) & 0xFFFFFFFFFFFFFFFF, int(first_run), int(num_runs),
+            self.M, nat.ptr(out), self.M, st), 'pc_permute_philox')
+        return out
+
+    # -- sweep ------------------------------------------------------------
+    def workspace_bytes(self, num_runs):
+        b = ctypes.c_uint64(0)
+        nat.check(self.lib.pc_sweep_workspace_size(self.N, num_runs,
+                                                   ctypes.byref(b)),
+                  'pc_sweep_workspace_size')
+        return int(b.value)
+
+    def sweep(self, d_perm, records=None, span_step=None, workspace=None):
+        torch = self.torch
+        R = int(d_perm.shape[0]) if self.M else int(d_perm.shape[0])
+        if records is None:
+            records = torch.empty((R, max(self.M, 1)), dtype=self.rec_torch,
+                                  device=self.device)
+        if span_step is None:
+            span_step = torch.empty((R,), dtype=torch.int32, device=self.device)
+        wb = self.workspace_bytes(R)
+        if wb and workspace is None:
+            workspace = torch.empty((wb,), dtype=torch.uint8, device=self.device)
+        st = nat.stream_handle(torch)
+        nat.check(self.lib.pc_sweep(
+            self.N, self.M, nat.ptr(self.d_edges), nat.ptr(self.d_sides),
+            self.span_first, nat.ptr(d_perm), max(self.M, 1), R,
+            nat.ptr(records), max(self.M, 1), nat.ptr(span_step),
+            nat.ptr(workspace), wb, st), 'pc_sweep')
+        return SweepResult(self, R, records, span_step, d_perm)
+
+    def chunk_scan(self, res):
+        torch = self.torch
+        C = max((self.M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
+        out = torch.empty((res.num_runs, C, nat.PC_STATE_WORDS),
+                          dtype=torch.int64, device=self.device)
+        st = nat.stream_handle(torch)
+        nat.check(self.lib.pc_chunk_scan(
+            self.N, self.M, res.num_runs, nat.ptr(res.records),
+            max(self.M, 1), nat.ptr(out), st), 'pc_chunk_scan')
+        return out
+
+    # -- outputs ------------------------------------------------------------
+    def microcanonical_sums(self, res):
+        """Exact per-step sums over runs: device tensors sum1 (7, S, 2) and
+        sum2 (7, S, 3) of uint64 limbs (stored as int64)."""
+        torch = self.torch
+        sum1 = torch.empty((nat.PC_COLUMNS, self.S, 2), dtype=torch.int64,
+                           device=self.device)
+        sum2 = torch.empty((nat.PC_COLUMNS, self.S, 3), dtype=torch.int64,
+                           device=self.device)
+        st = nat.stream_handle(torch)
+        nat.check(self.lib.pc_microcanonical_sums(
+            self.N, self.M, res.num_runs, nat.ptr(res.records),
+            max(self.M, 1), nat.ptr(res.span_step), nat.ptr(res.chunk_state),
+            nat.ptr(sum1), nat.ptr(sum2), st), 'pc_microcanonical_sums')
+        return sum1, sum2
+
+    def traces(self, res):
+        """Per-run traces on the device: max (R, S) int32 (u32 bits),
+        spanning (R, S) uint8, moments (R, S, 5) int64 (u64 bits)."""
+        torch = self.torch
+        R = res.num_runs
+        mx = torch.empty((R, self.S), dtype=torch.int32, device=self.device)
+        sp = torch.empty((R, self.S), dtype=torch.uint8, device=self.device)
+        mom = torch.empty((R, self.S, 5), dtype=torch.int64, device=self.device)
+        st = nat.stream_handle(torch)
+        nat.check(self.lib.pc_expand_traces(
+            self.N, self.M, R, nat.ptr(res.records), max(self.M, 1),
+            nat.ptr(res.span_step), nat.ptr(res.chunk_state), nat.ptr(mx),
+            nat.ptr(sp), nat.ptr(mom), st), 'pc_expand_traces')
+        return mx, sp, mom
+
+    def host_traces(self, res):
+        """Per-run traces copied to numpy: max u32, spanning bool, moments u64."""
+        mx, sp, mom = self.traces(res)
+        return (mx.cpu().numpy().view(np.uint32), sp.cpu().numpy().astype(bool),
+                mom.cpu().numpy().view(np.uint64))
+
+
+def limbs_to_int(limbs):
+    """(..., L) uint64 limb array (little-endian) -> object array of ints."""
+    limbs = np.asarray(limbs).view(np.uint64)
+    out = np.zeros(limbs.shape[:-1], dtype=object)
+    for j in range(limbs.shape[-1] - 1, -1, -1):
+        out = out * (1 << 64) + limbs[..., j].astype(object)
+    return out
+
+
+def host_sums(sum1, sum2):
+    """Device sum tensors -> exact Python-int object arrays (7, S)."""
+    return (limbs_to_int(sum1.cpu().numpy()),
+            limbs_to_int(sum2.cpu().numpy()))

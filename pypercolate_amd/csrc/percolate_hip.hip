@@ -1,0 +1,1027 @@
+// libpercolate_hip: MI355X (gfx950, wave64, CDNA4) kernels for the
+// Newman-Ziff bond-percolation hot path, behind the C ABI declared in
+// include/percolate_hip.h.
+//
+// Pipeline for R runs on a graph with N nodes and M edges:
+//   permute (Philox-keyed Feistel, optional) -> sweep (union-find, one run per
+//   wavefront, merge records) -> chunk_scan (per-run state every 64 steps) ->
+//   microcanonical sums (exact Sigma x, Sigma x^2 over runs per step) or
+//   expand (per-run traces).
+//
+// The observables of one run at step s (largest cluster, spanning flag,
+// moments k = 0..4 of all clusters but the largest) depend only on the set
+// of bonds occupied after s steps.  With Sk(s) = sum over all clusters of
+// size**k they are exact prefix functions of the merge records (a, b):
+//   max(s) = max(1, max over merges <= s of a + b)
+//   #clusters(s) = N - merges(s)
+//   Sk(s) = N + sum over merges <= s of (a + b)**k - a**k - b**k
+//   moments_k(s) = Sk(s) - max(s)**k      (k = 0: #clusters - 1)
+// which equals the reference's incremental bookkeeping
+// (percolate/hpc.py:282-305) whichever largest cluster it excludes on ties.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <type_traits>
+
+#include "percolate_hip.h"
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------
+// Error reporting
+// ---------------------------------------------------------------------------
+static thread_local char g_last_error[512];
+
+static int pc_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define PC_HIP_CHECK(expr)                                                     \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return pc_fail(PC_EHIP, "%s failed: %s", #expr,                    \
+                           hipGetErrorString(e_));                             \
+    } while (0)
+
+#define PC_LAUNCH_CHECK(name)                                                  \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess)                                                  \
+            return pc_fail(PC_EHIP, "launch of %s failed: %s", name,           \
+                           hipGetErrorString(e_));                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Wavefront helpers (wave64).  Scans use DPP row shifts inside each row of
+// 16 lanes and the GFX9 row broadcasts 15 / 31 across rows.
+// ---------------------------------------------------------------------------
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v)
+{
+    // lanes without a source (row edge or masked row) read 0, the identity
+    // of every scan below (sum and unsigned max)
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK,
+                                                 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v)
+{
+    return (uint64_t)dpp32<CTRL, ROW_MASK>((uint32_t)v) |
+           ((uint64_t)dpp32<CTRL, ROW_MASK>((uint32_t)(v >> 32)) << 32);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ u128 dpp128(u128 v)
+{
+    return (u128)dpp64<CTRL, ROW_MASK>((uint64_t)v) |
+           ((u128)dpp64<CTRL, ROW_MASK>((uint64_t)(v >> 64)) << 64);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_any(uint32_t v) { return dpp32<CTRL, ROW_MASK>(v); }
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint64_t dpp_any(uint64_t v) { return dpp64<CTRL, ROW_MASK>(v); }
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ u128 dpp_any(u128 v) { return dpp128<CTRL, ROW_MASK>(v); }
+
+// Inclusive prefix sum over the 64 lanes.
+template <typename T>
+__device__ __forceinline__ T wave_scan_add(T x)
+{
+    x += dpp_any<DPP_ROW_SHR(1)>(x);
+    x += dpp_any<DPP_ROW_SHR(2)>(x);
+    x += dpp_any<DPP_ROW_SHR(4)>(x);
+    x += dpp_any<DPP_ROW_SHR(8)>(x);
+    x += dpp_any<DPP_ROW_BCAST15, 0xa>(x);
+    x += dpp_any<DPP_ROW_BCAST31, 0xc>(x);
+    return x;
+}
+
+// Inclusive prefix maximum over the 64 lanes (unsigned, identity 0).
+__device__ __forceinline__ uint32_t wave_scan_max(uint32_t x)
+{
+    x = max(x, dpp32<DPP_ROW_SHR(1)>(x));
+    x = max(x, dpp32<DPP_ROW_SHR(2)>(x));
+    x = max(x, dpp32<DPP_ROW_SHR(4)>(x));
+    x = max(x, dpp32<DPP_ROW_SHR(8)>(x));
+    x = max(x, dpp32<DPP_ROW_BCAST15, 0xa>(x));
+    x = max(x, dpp32<DPP_ROW_BCAST31, 0xc>(x));
+    return x;
+}
+
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+__device__ __forceinline__ uint64_t read_lane(uint64_t v, uint32_t lane)
+{
+    return (uint64_t)read_lane((uint32_t)v, lane) |
+           ((uint64_t)read_lane((uint32_t)(v >> 32), lane) << 32);
+}
+__device__ __forceinline__ u128 read_lane(u128 v, uint32_t lane)
+{
+    return (u128)read_lane((uint64_t)v, lane) |
+           ((u128)read_lane((uint64_t)(v >> 64), lane) << 64);
+}
+
+// Multi-word atomic adds: each word's carry-out is recovered from the old
+// value the atomic returns, so concurrent adders compose exactly.
+__device__ __forceinline__ void atomic_add_u128(uint64_t *p, u128 v)
+{
+    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+    if (lo) {
+        unsigned long long old = atomicAdd((unsigned long long *)p, lo);
+        if (old + lo < old)
+            hi += 1;
+    }
+    if (hi)
+        atomicAdd((unsigned long long *)(p + 1), hi);
+}
+
+__device__ __forceinline__ void atomic_add_u192(uint64_t *p, uint64_t w0,
+                                                uint64_t w1, uint64_t w2)
+{
+    if (w0) {
+        unsigned long long old = atomicAdd((unsigned long long *)p, w0);
+        if (old + w0 < old) {
+            w1 += 1;
+            if (w1 == 0)
+                w2 += 1;
+        }
+    }
+    if (w1) {
+        unsigned long long old = atomicAdd((unsigned long long *)(p + 1), w1);
+        if (old + w1 < old)
+            w2 += 1;
+    }
+    if (w2)
+        atomicAdd((unsigned long long *)(p + 2), w2);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used as the keyed PRF
+// that fills the Feistel round tables.
+// ---------------------------------------------------------------------------
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
+                                               uint32_t k1)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        if (i) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = u32x4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1,
+                  (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// Device permutations: a balanced Feistel network on 2h bits (2^(2h) >= M)
+// with PERM_ROUNDS random round tables T_j : [0, 2^h) -> [0, 2^h), drawn
+// per run from Philox4x32-10(counter = (table word, run lo, run hi, tag),
+// key = seed).  Cycle walking restricts the bijection of [0, 2^(2h)) to
+// [0, M).  perm[i] = edge index added at step i + 1.
+// ---------------------------------------------------------------------------
+#define PERM_ROUNDS 8
+#define PERM_BLOCK 256
+#define PERM_ITEMS 16
+#define PERM_TAG 0x7065726du
+
+__global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
+    uint32_t seed_lo, uint32_t seed_hi, uint64_t first_run, uint32_t M,
+    uint32_t h, int32_t *__restrict__ perm, int64_t stride)
+{
+    extern __shared__ uint16_t table[];
+    const uint64_t run = first_run + blockIdx.y;
+    const uint32_t entries = PERM_ROUNDS << h;
+    const uint32_t mask = (1u << h) - 1u;
+
+    for (uint32_t g = threadIdx.x; g * 8u < entries; g += PERM_BLOCK) {
+        const u32x4 o = philox4x32_10(
+            u32x4{g, (uint32_t)run, (uint32_t)(run >> 32), PERM_TAG}, seed_lo,
+            seed_hi);
+        const uint32_t words[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t e = g * 8u + (uint32_t)j;
+            if (e < entries)
+                table[e] = (uint16_t)((words[j >> 1] >> (16 * (j & 1))) & mask);
+        }
+    }
+    __syncthreads();
+
+    int32_t *out = perm + (int64_t)blockIdx.y * stride;
+    const uint32_t base = blockIdx.x * (PERM_BLOCK * PERM_ITEMS);
+#pragma unroll 4
+    for (int k = 0; k < PERM_ITEMS; ++k) {
+        const uint32_t i = base + (uint32_t)k * PERM_BLOCK + threadIdx.x;
+        if (i >= M)
+            break;
+        uint32_t y = i;
+        do {
+            uint32_t L = y >> h, R = y & mask;
+#pragma unroll
+            for (int j = 0; j < PERM_ROUNDS; ++j) {
+                const uint32_t t = L ^ table[((uint32_t)j << h) | R];
+                L = R;
+                R = t;
+            }
+            y = (L << h) | R;
+        } while (y >= M);
+        out[i] = (int32_t)y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Union-find state.  A node's code is either its parent's index (non-root)
+// or, for a root, the cluster size and which spanning side (0 none, 1 first,
+// 2 second) the cluster touches.  Touching both sides is recorded once as
+// the run's spanning step and not stored (spanning is monotone).
+// ---------------------------------------------------------------------------
+
+// 16-bit codes in LDS for N <= 16384: c < N parent; c >= N root with
+// c - N = f * N + (size - 1).  The largest code, 4N - 1, fits 16 bits.
+struct LdsState {
+    uint16_t *code;
+    uint32_t N;
+    __device__ __forceinline__ uint32_t get(uint32_t i) const { return code[i]; }
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { code[i] = (uint16_t)v; }
+    __device__ __forceinline__ bool is_root(uint32_t v) const { return v >= N; }
+    __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return N * (1u + f) + size - 1u; }
+    __device__ __forceinline__ void decode(uint32_t v, uint32_t &size, uint32_t &f) const
+    {
+        const uint32_t j = v - N;
+        f = (uint32_t)(j >= N) + (uint32_t)(j >= 2u * N);
+        size = j - f * N + 1u;
+    }
+};
+
+// 32-bit codes in HBM for larger N: bit 31 marks a root, bits 29-30 the
+// side, bits 0-28 the size.
+struct GlobalState {
+    uint32_t *code;
+    __device__ __forceinline__ uint32_t get(uint32_t i) const { return code[i]; }
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { code[i] = v; }
+    __device__ __forceinline__ bool is_root(uint32_t v) const { return (v >> 31) != 0u; }
+    __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return 0x80000000u | (f << 29) | size; }
+    __device__ __forceinline__ void decode(uint32_t v, uint32_t &size, uint32_t &f) const
+    {
+        size = v & 0x1FFFFFFFu;
+        f = (v >> 29) & 3u;
+    }
+};
+
+template <typename RecT>
+__device__ __forceinline__ RecT pack_record(uint32_t a, uint32_t b);
+template <>
+__device__ __forceinline__ uint32_t pack_record<uint32_t>(uint32_t a, uint32_t b) { return a | (b << 16); }
+template <>
+__device__ __forceinline__ uint64_t pack_record<uint64_t>(uint32_t a, uint32_t b) { return (uint64_t)a | ((uint64_t)b << 32); }
+
+__device__ __forceinline__ void unpack_record(uint32_t r, uint32_t &a, uint32_t &b)
+{
+    a = r & 0xFFFFu;
+    b = r >> 16;
+}
+__device__ __forceinline__ void unpack_record(uint64_t r, uint32_t &a, uint32_t &b)
+{
+    a = (uint32_t)r;
+    b = (uint32_t)(r >> 32);
+}
+
+// Two interleaved finds with path halving.  Concurrent halving by other
+// lanes of the wave only ever replaces a parent by an ancestor, so every
+// lane still reaches its true root.
+template <class St>
+__device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
+                                      uint32_t &y, uint32_t &cy)
+{
+    cx = st.get(x);
+    cy = st.get(y);
+    bool rx = st.is_root(cx), ry = st.is_root(cy);
+    while (!(rx && ry)) {
+        const uint32_t gx = rx ? cx : st.get(cx);
+        const uint32_t gy = ry ? cy : st.get(cy);
+        if (!rx) {
+            if (st.is_root(gx)) {
+                x = cx;
+                cx = gx;
+                rx = true;
+            } else {
+                st.set(x, gx);
+                x = gx;
+                cx = st.get(x);
+                rx = st.is_root(cx);
+            }
+        }
+        if (!ry) {
+            if (st.is_root(gy)) {
+                y = cy;
+                cy = gy;
+                ry = true;
+            } else {
+                st.set(y, gy);
+                y = gy;
+                cy = st.get(y);
+                ry = st.is_root(cy);
+            }
+        }
+    }
+}
+
+// One run on one wavefront.  Bonds are taken 64 at a time (lane = step):
+//  1. every lane finds the roots of its bond's endpoints in parallel;
+//  2. lanes whose roots differ are candidate merges; they are resolved in
+//     step order on scalar registers (readlane), and after each merge every
+//     lane relabels its cached roots, so later candidates see the merge;
+//  3. the new parent pointers and root codes are written back in parallel.
+// Per-step output is one merge record; the spanning step is kept scalar.
+template <class St, typename RecT>
+__device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
+                                          const int2 *__restrict__ edges,
+                                          const uint8_t *__restrict__ sides,
+                                          bool span_first,
+                                          const int32_t *__restrict__ perm,
+                                          RecT *__restrict__ rec_out,
+                                          uint32_t *__restrict__ span_out)
+{
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < N; i += 64u)
+        st.set(i, st.root(1u, sides ? (uint32_t)sides[i] : 0u));
+    __syncthreads();
+
+    uint32_t span_at = PC_NO_SPAN;
+
+    // software pipeline: the permutation two batches ahead and the edge
+    // endpoints one batch ahead are in flight while a batch is resolved
+    int32_t e_ahead = (64u + lane < M) ? perm[64u + lane] : 0;
+    int2 uv_next = make_int2(0, 0);
+    if (lane < M)
+        uv_next = edges[perm[lane]];
+
+    for (uint32_t base = 0; base < M; base += 64u) {
+        const uint32_t n = base + lane;
+        const bool active = n < M;
+        const int2 uv = uv_next;
+        uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
+        e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
+
+        uint32_t ru = (uint32_t)uv.x, rv = (uint32_t)uv.y, cu, cv;
+        find2(st, ru, cu, rv, cv);
+        const bool cand = active && ru != rv;
+        uint64_t pending = __ballot(cand);
+
+        RecT rec = 0;
+        uint32_t small_l = 0, big_l = 0;
+        bool merged = false;
+        while (pending) {
+            const uint32_t i = (uint32_t)__builtin_ctzll(pending);
+            pending &= pending - 1u;
+            const uint32_t a = read_lane(ru, i), b = read_lane(rv, i);
+            if (a == b)
+                continue; // joined by an earlier bond of this batch
+            uint32_t sa, fa, sb, fb;
+            st.decode(read_lane(cu, i), sa, fa);
+            st.decode(read_lane(cv, i), sb, fb);
+            const bool a_big = sa >= sb;
+            const uint32_t big = a_big ? a : b, small = a_big ? b : a;
+            uint32_t f = fa | fb;
+            if (f == 3u || span_first)
+                span_at = min(span_at, base + i + 1u);
+            if (f == 3u)
+                f = 0u;
+            const uint32_t nc = st.root(sa + sb, f);
+            ru = (ru == small) ? big : ru;
+            cu = (ru == big) ? nc : cu;
+            rv = (rv == small) ? big : rv;
+            cv = (rv == big) ? nc : cv;
+            if (lane == i) {
+                rec = pack_record<RecT>(sa, sb);
+                small_l = small;
+                big_l = big;
+                merged = true;
+            }
+        }
+        // pointer writes go to the roots merged away in this batch, root
+        // writes to the surviving roots: disjoint addresses, and lanes that
+        // share a root write the same value
+        if (merged)
+            st.set(small_l, big_l);
+        if (cand) {
+            st.set(ru, cu);
+            st.set(rv, cv);
+        }
+        if (active)
+            rec_out[n] = rec;
+        __syncthreads();
+    }
+    if (lane == 0)
+        *span_out = span_at;
+}
+
+template <typename RecT>
+__global__ __launch_bounds__(64) void sweep_lds_kernel(
+    uint32_t N, uint32_t M, const int2 *__restrict__ edges,
+    const uint8_t *__restrict__ sides, int span_first,
+    const int32_t *__restrict__ perm, int64_t perm_stride,
+    RecT *__restrict__ records, int64_t rec_stride,
+    uint32_t *__restrict__ span_step)
+{
+    extern __shared__ uint16_t lds_code[];
+    const uint32_t r = blockIdx.x;
+    const LdsState st{lds_code, N};
+    sweep_run(st, N, M, edges, sides, span_first != 0,
+              perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
+              span_step + r);
+}
+
+template <typename RecT>
+__global__ __launch_bounds__(64) void sweep_global_kernel(
+    uint32_t N, uint32_t M, uint32_t R, const int2 *__restrict__ edges,
+    const uint8_t *__restrict__ sides, int span_first,
+    const int32_t *__restrict__ perm, int64_t perm_stride,
+    RecT *__restrict__ records, int64_t rec_stride,
+    uint32_t *__restrict__ span_step, uint32_t *__restrict__ workspace)
+{
+    const GlobalState st{workspace + (size_t)blockIdx.x * N};
+    for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
+        sweep_run(st, N, M, edges, sides, span_first != 0,
+                  perm + (int64_t)r * perm_stride,
+                  records + (int64_t)r * rec_stride, span_step + r);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge-record arithmetic.  WIDE (N > 16384) keeps S3, S4 and the moments in
+// 128 bits; otherwise every Sk fits 64 bits (Sk <= N**k <= 2**56).
+// ---------------------------------------------------------------------------
+template <bool WIDE>
+struct Wide {
+    using T = typename std::conditional<WIDE, u128, uint64_t>::type;
+};
+
+template <bool WIDE>
+struct Deltas {
+    using MT = typename Wide<WIDE>::T;
+    uint32_t dmax; // size of the merged cluster (0: no merge)
+    uint32_t cnt;  // 1 for a merge
+    uint64_t d2;   // (a+b)^2 - a^2 - b^2
+    MT d3, d4;
+    template <typename RecT>
+    __device__ __forceinline__ explicit Deltas(RecT rec)
+    {
+        uint32_t a, b;
+        unpack_record(rec, a, b);
+        const bool m = rec != 0;
+        dmax = m ? a + b : 0u;
+        cnt = m ? 1u : 0u;
+        const uint64_t ab = (uint64_t)a * b;
+        const uint64_t aa = (uint64_t)a * a, bb = (uint64_t)b * b;
+        d2 = 2u * ab;
+        d3 = (MT)3u * (MT)ab * (MT)(a + b);
+        d4 = (MT)(2u * ab) * (MT)(2u * aa + 3u * ab + 2u * bb);
+    }
+};
+
+// Running per-run state (after some number of bonds).
+template <bool WIDE>
+struct RunState {
+    using MT = typename Wide<WIDE>::T;
+    uint64_t mx, merges, s2;
+    MT s3, s4;
+};
+
+template <bool WIDE>
+__device__ __forceinline__ RunState<WIDE> load_state(const uint64_t *cs)
+{
+    RunState<WIDE> st;
+    st.mx = cs[0];
+    st.merges = cs[1];
+    st.s2 = cs[2];
+    if constexpr (WIDE) {
+        st.s3 = (u128)cs[3] | ((u128)cs[4] << 64);
+        st.s4 = (u128)cs[5] | ((u128)cs[6] << 64);
+    } else {
+        st.s3 = cs[3];
+        st.s4 = cs[5];
+    }
+    return st;
+}
+
+// State after each lane's step: chunk-start state plus the inclusive scan
+// of the deltas of the chunk's records.
+template <bool WIDE>
+__device__ __forceinline__ RunState<WIDE> advance(const RunState<WIDE> &st0,
+                                                  const Deltas<WIDE> &d)
+{
+    RunState<WIDE> st;
+    st.mx = umax64(st0.mx, (uint64_t)wave_scan_max(d.dmax));
+    st.merges = st0.merges + wave_scan_add(d.cnt);
+    st.s2 = st0.s2 + wave_scan_add(d.d2);
+    st.s3 = st0.s3 + wave_scan_add(d.d3);
+    st.s4 = st0.s4 + wave_scan_add(d.d4);
+    return st;
+}
+
+// The seven observables from a state: x[0] max, x[1] spanning, x[2..6]
+// moments k = 0..4 of all clusters but one largest.
+template <bool WIDE>
+__device__ __forceinline__ void observables(const RunState<WIDE> &st,
+                                            uint32_t N, bool span,
+                                            typename Wide<WIDE>::T x[7])
+{
+    using MT = typename Wide<WIDE>::T;
+    const uint64_t m = st.mx;
+    x[0] = (MT)m;
+    x[1] = (MT)(span ? 1u : 0u);
+    x[2] = (MT)((uint64_t)N - st.merges - 1u);
+    x[3] = (MT)((uint64_t)N - m);
+    const MT m2 = (MT)(m * m);
+    x[4] = (MT)st.s2 - m2;
+    x[5] = st.s3 - m2 * (MT)m;
+    x[6] = st.s4 - m2 * m2;
+}
+
+// ---------------------------------------------------------------------------
+// chunk_scan: one wavefront per run walks its records 64 at a time.
+// ---------------------------------------------------------------------------
+template <typename RecT, bool WIDE>
+__global__ __launch_bounds__(64) void chunk_scan_kernel(
+    uint32_t N, uint32_t M, const RecT *__restrict__ records,
+    int64_t rec_stride, uint64_t *__restrict__ chunk_state)
+{
+    using MT = typename Wide<WIDE>::T;
+    const uint32_t r = blockIdx.x, lane = threadIdx.x;
+    const uint32_t C = (M + 63u) / 64u;
+    const RecT *rec_r = records + (int64_t)r * rec_stride;
+    uint64_t *out = chunk_state + (size_t)r * C * PC_STATE_WORDS;
+
+    uint64_t mx = 1, merges = 0, s2 = N;
+    MT s3 = N, s4 = N;
+    for (uint32_t c = 0; c < C; ++c) {
+        if (lane < PC_STATE_WORDS) {
+            uint64_t w = 0;
+            switch (lane) {
+            case 0: w = mx; break;
+            case 1: w = merges; break;
+            case 2: w = s2; break;
+            case 3: w = (uint64_t)s3; break;
+            case 4: w = (uint64_t)((u128)s3 >> 64); break;
+            case 5: w = (uint64_t)s4; break;
+            case 6: w = (uint64_t)((u128)s4 >> 64); break;
+            default: break;
+            }
+            out[(size_t)c * PC_STATE_WORDS + lane] = w;
+        }
+        const uint32_t q = c * 64u + lane;
+        const RecT rec = q < M ? rec_r[q] : (RecT)0;
+        const Deltas<WIDE> d(rec);
+        mx = umax64(mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
+        merges += read_lane(wave_scan_add(d.cnt), 63u);
+        s2 += read_lane(wave_scan_add(d.d2), 63u);
+        s3 += read_lane(wave_scan_add(d.d3), 63u);
+        s4 += read_lane(wave_scan_add(d.d4), 63u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Microcanonical sums: block (chunk c, run group g), lane = step.  Each lane
+// accumulates its step's sums over the group's runs in registers; groups
+// combine through multi-word atomics.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void square_u192(u128 x, uint64_t &w0, uint64_t &w1,
+                                            uint64_t &w2)
+{
+    const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    const u128 ll = (u128)lo * lo;
+    const u128 lh2 = ((u128)lo * hi) << 1; // < 2^129 only if hi >= 2^64: hi < 2^32 here
+    const u128 hh = (u128)hi * hi;
+    w0 = (uint64_t)ll;
+    const u128 mid = (ll >> 64) + (u128)(uint64_t)lh2;
+    w1 = (uint64_t)mid;
+    w2 = (uint64_t)(mid >> 64) + (uint64_t)(lh2 >> 64) + (uint64_t)hh;
+}
+
+template <typename RecT, bool WIDE>
+__global__ __launch_bounds__(64) void micro_sums_kernel(
+    uint32_t N, uint32_t M, uint32_t R, uint32_t runs_per_group,
+    const RecT *__restrict__ records, int64_t rec_stride,
+    const uint32_t *__restrict__ span_step,
+    const uint64_t *__restrict__ chunk_state, uint64_t *__restrict__ sum1,
+    uint64_t *__restrict__ sum2)
+{
+    using MT = typename Wide<WIDE>::T;
+    const uint32_t c = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
+    const uint32_t C = (M + 63u) / 64u;
+    const uint64_t S = (uint64_t)M + 1u;
+    const uint32_t q = c * 64u + lane, s = q + 1u;
+    const bool valid = q < M;
+    const uint32_t r0 = g * runs_per_group;
+    const uint32_t r1 = min(R, r0 + runs_per_group);
+
+    u128 acc1[7];
+    u128 acc2[7];
+    uint64_t acc2_top[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        acc1[k] = 0;
+        acc2[k] = 0;
+        acc2_top[k] = 0;
+    }
+
+    for (uint32_t r = r0; r < r1; ++r) {
+        const RunState<WIDE> st0 = load_state<WIDE>(
+            chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS);
+        const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+        const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
+        MT x[7];
+        observables<WIDE>(st, N, s >= span_step[r], x);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            acc1[k] += (u128)x[k];
+            if constexpr (WIDE) {
+                uint64_t w0, w1, w2;
+                square_u192((u128)x[k], w0, w1, w2);
+                const u128 low = (u128)w0 | ((u128)w1 << 64);
+                const u128 before = acc2[k];
+                acc2[k] += low;
+                acc2_top[k] += w2 + (acc2[k] < before ? 1u : 0u);
+            } else {
+                acc2[k] += (u128)x[k] * (u128)x[k];
+            }
+        }
+    }
+
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            atomic_add_u128(sum1 + ((uint64_t)k * S + s) * 2u, acc1[k]);
+            atomic_add_u192(sum2 + ((uint64_t)k * S + s) * 3u,
+                            (uint64_t)acc2[k], (uint64_t)(acc2[k] >> 64),
+                            acc2_top[k]);
+        }
+    }
+    if (c == 0 && lane == 0 && r1 > r0) {
+        // step 0: max 1, no spanning, every moment N - 1
+        const u128 n = r1 - r0;
+        const u128 x0[7] = {1, 0, N - 1u, N - 1u, N - 1u, N - 1u, N - 1u};
+        for (int k = 0; k < 7; ++k) {
+            atomic_add_u128(sum1 + (uint64_t)k * S * 2u, n * x0[k]);
+            const u128 sq = n * x0[k] * x0[k];
+            atomic_add_u192(sum2 + (uint64_t)k * S * 3u, (uint64_t)sq,
+                            (uint64_t)(sq >> 64), 0u);
+        }
+    }
+}
+
+// Step 0 alone (graphs without edges): max 1, no spanning, moments N - 1.
+__global__ void step0_sums_kernel(uint32_t N, uint32_t R, uint64_t S,
+                                  uint64_t *__restrict__ sum1,
+                                  uint64_t *__restrict__ sum2)
+{
+    const u128 n = R;
+    const u128 x0[7] = {1, 0, N - 1u, N - 1u, N - 1u, N - 1u, N - 1u};
+    for (int k = 0; k < 7; ++k) {
+        atomic_add_u128(sum1 + (uint64_t)k * S * 2u, n * x0[k]);
+        const u128 sq = n * x0[k] * x0[k];
+        atomic_add_u192(sum2 + (uint64_t)k * S * 3u, (uint64_t)sq,
+                        (uint64_t)(sq >> 64), 0u);
+    }
+}
+
+__global__ void expand_step0_kernel(uint32_t N, uint32_t R, uint64_t S,
+                                    uint32_t *__restrict__ out_max,
+                                    uint8_t *__restrict__ out_span,
+                                    uint64_t *__restrict__ out_mom)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R)
+        return;
+    const uint64_t row = (uint64_t)r * S;
+    out_max[row] = 1u;
+    out_span[row] = 0u;
+    for (int k = 0; k < 5; ++k)
+        out_mom[row * 5u + k] = (uint64_t)N - 1u;
+}
+
+__global__ void fill_zero_kernel(uint64_t *__restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Per-run traces: block (chunk c, run r), lane = step.
+// ---------------------------------------------------------------------------
+template <typename RecT, bool WIDE>
+__global__ __launch_bounds__(64) void expand_kernel(
+    uint32_t N, uint32_t M, const RecT *__restrict__ records,
+    int64_t rec_stride, const uint32_t *__restrict__ span_step,
+    const uint64_t *__restrict__ chunk_state, uint32_t *__restrict__ out_max,
+    uint8_t *__restrict__ out_span, uint64_t *__restrict__ out_mom)
+{
+    using MT = typename Wide<WIDE>::T;
+    const uint32_t c = blockIdx.x, r = blockIdx.y, lane = threadIdx.x;
+    const uint32_t C = (M + 63u) / 64u;
+    const uint64_t S = (uint64_t)M + 1u;
+    const uint32_t q = c * 64u + lane, s = q + 1u;
+    const bool valid = q < M;
+    const RunState<WIDE> st0 = load_state<WIDE>(
+        chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS);
+    const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+    const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
+    MT x[7];
+    observables<WIDE>(st, N, s >= span_step[r], x);
+    const uint64_t row = (uint64_t)r * S + s;
+    if (valid) {
+        out_max[row] = (uint32_t)x[0];
+        out_span[row] = (uint8_t)x[1];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            out_mom[row * 5u + k] = (uint64_t)x[2 + k];
+    }
+    if (c == 0 && lane == 0) {
+        const uint64_t row0 = (uint64_t)r * S;
+        out_max[row0] = 1u;
+        out_span[row0] = 0u;
+        for (int k = 0; k < 5; ++k)
+            out_mom[row0 * 5u + k] = (uint64_t)N - 1u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static const uint32_t kLdsMaxNodes = 16384u;
+static const uint32_t kU32RecordMaxNodes = 65536u;
+static const uint32_t kSweepMaxNodes = 1u << 28;
+static const uint32_t kSumsMaxNodes = 1u << 21;
+static const uint32_t kGlobalSlots = 4096u;
+
+static inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+extern "C" int pc_abi_version(void) { return 1; }
+
+extern "C" const char *pc_last_error(void) { return g_last_error; }
+
+extern "C" int pc_record_bytes(uint32_t num_nodes)
+{
+    return num_nodes <= kU32RecordMaxNodes ? 4 : 8;
+}
+
+extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
+                                 uint32_t num_runs, uint32_t num_edges,
+                                 int32_t *d_perm, int64_t perm_stride,
+                                 void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0 || num_edges == 0)
+        return PC_OK;
+    if (!d_perm || perm_stride < (int64_t)num_edges)
+        return pc_fail(PC_EINVAL, "pc_permute_philox: bad output buffer");
+    if (num_edges > (1u << 24))
+        return pc_fail(PC_EOVERFLOW, "pc_permute_philox: num_edges > 2^24");
+    uint32_t bits = 1;
+    while ((1ull << bits) < num_edges)
+        ++bits;
+    const uint32_t h = (bits + 1u) / 2u;
+    const size_t lds = (size_t)(PERM_ROUNDS << h) * sizeof(uint16_t);
+    const dim3 grid((num_edges + PERM_BLOCK * PERM_ITEMS - 1) /
+                        (PERM_BLOCK * PERM_ITEMS),
+                    num_runs);
+    permute_kernel<<<grid, PERM_BLOCK, lds, as_stream(stream)>>>(
+        (uint32_t)seed, (uint32_t)(seed >> 32), first_run, num_edges, h,
+        d_perm, perm_stride);
+    PC_LAUNCH_CHECK("permute_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
+                                       uint64_t *bytes)
+{
+    g_last_error[0] = 0;
+    if (!bytes)
+        return pc_fail(PC_EINVAL, "pc_sweep_workspace_size: bytes is NULL");
+    if (num_nodes <= kLdsMaxNodes) {
+        *bytes = 0;
+    } else {
+        const uint64_t slots = num_runs < kGlobalSlots ? num_runs : kGlobalSlots;
+        *bytes = slots * (uint64_t)num_nodes * sizeof(uint32_t);
+    }
+    return PC_OK;
+}
+
+extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
+                        const int32_t *d_edges, const uint8_t *d_node_sides,
+                        int32_t span_on_first_merge, const int32_t *d_perm,
+                        int64_t perm_stride, uint32_t num_runs, void *d_records,
+                        int64_t rec_stride, uint32_t *d_span_step,
+                        void *d_workspace, uint64_t workspace_bytes,
+                        void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0)
+        return PC_OK;
+    if (num_nodes == 0)
+        return pc_fail(PC_EINVAL, "pc_sweep: num_nodes must be positive");
+    if (num_nodes > kSweepMaxNodes)
+        return pc_fail(PC_EOVERFLOW, "pc_sweep: num_nodes > 2^28");
+    if (!d_span_step || (num_edges && (!d_edges || !d_perm || !d_records)))
+        return pc_fail(PC_EINVAL, "pc_sweep: NULL buffer");
+    if (num_edges && (perm_stride < (int64_t)num_edges ||
+                      rec_stride < (int64_t)num_edges))
+        return pc_fail(PC_EINVAL, "pc_sweep: stride smaller than num_edges");
+    const int2 *edges = (const int2 *)d_edges;
+    hipStream_t s = as_stream(stream);
+    const bool u32rec = num_nodes <= kU32RecordMaxNodes;
+    if (num_nodes <= kLdsMaxNodes) {
+        const size_t lds = ((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u;
+        sweep_lds_kernel<uint32_t><<<num_runs, 64, lds, s>>>(
+            num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
+            d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
+            d_span_step);
+        PC_LAUNCH_CHECK("sweep_lds_kernel");
+        return PC_OK;
+    }
+    uint64_t need = 0;
+    pc_sweep_workspace_size(num_nodes, num_runs, &need);
+    if (!d_workspace || workspace_bytes < (uint64_t)num_nodes * 4u)
+        return pc_fail(PC_ENOMEM,
+                       "pc_sweep: workspace of %llu bytes too small (need %llu)",
+                       (unsigned long long)workspace_bytes,
+                       (unsigned long long)need);
+    uint64_t slots = workspace_bytes / ((uint64_t)num_nodes * 4u);
+    if (slots > num_runs)
+        slots = num_runs;
+    if (slots > kGlobalSlots)
+        slots = kGlobalSlots;
+    if (u32rec) {
+        sweep_global_kernel<uint32_t><<<(uint32_t)slots, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, edges, d_node_sides,
+            span_on_first_merge, d_perm, perm_stride, (uint32_t *)d_records,
+            rec_stride, d_span_step, (uint32_t *)d_workspace);
+    } else {
+        sweep_global_kernel<uint64_t><<<(uint32_t)slots, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, edges, d_node_sides,
+            span_on_first_merge, d_perm, perm_stride, (uint64_t *)d_records,
+            rec_stride, d_span_step, (uint32_t *)d_workspace);
+    }
+    PC_LAUNCH_CHECK("sweep_global_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_chunk_scan(uint32_t num_nodes, uint32_t num_edges,
+                             uint32_t num_runs, const void *d_records,
+                             int64_t rec_stride, uint64_t *d_chunk_state,
+                             void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0 || num_edges == 0)
+        return PC_OK;
+    if (num_nodes == 0 || num_nodes > kSweepMaxNodes)
+        return pc_fail(PC_EOVERFLOW, "pc_chunk_scan: num_nodes out of range");
+    if (!d_records || !d_chunk_state || rec_stride < (int64_t)num_edges)
+        return pc_fail(PC_EINVAL, "pc_chunk_scan: bad buffer");
+    hipStream_t s = as_stream(stream);
+    if (num_nodes <= kLdsMaxNodes)
+        chunk_scan_kernel<uint32_t, false><<<num_runs, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
+            d_chunk_state);
+    else if (num_nodes <= kU32RecordMaxNodes)
+        chunk_scan_kernel<uint32_t, true><<<num_runs, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
+            d_chunk_state);
+    else
+        chunk_scan_kernel<uint64_t, true><<<num_runs, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint64_t *)d_records, rec_stride,
+            d_chunk_state);
+    PC_LAUNCH_CHECK("chunk_scan_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
+                                      uint32_t num_runs, const void *d_records,
+                                      int64_t rec_stride,
+                                      const uint32_t *d_span_step,
+                                      const uint64_t *d_chunk_state,
+                                      uint64_t *d_sum1, uint64_t *d_sum2,
+                                      void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_nodes == 0 || num_nodes > kSumsMaxNodes)
+        return pc_fail(PC_EOVERFLOW,
+                       "pc_microcanonical_sums: num_nodes outside 1..2^21");
+    if (num_nodes <= kLdsMaxNodes) {
+        // the 128-bit sums of squares hold R * (N^4 / 8)^2
+        const double n4 = (double)num_nodes * num_nodes * num_nodes * num_nodes / 8.0;
+        if ((double)num_runs * n4 * n4 >= 0x1p127)
+            return pc_fail(PC_EOVERFLOW,
+                           "pc_microcanonical_sums: too many runs for N = %u",
+                           num_nodes);
+    }
+    if (!d_sum1 || !d_sum2 || (num_runs && num_edges &&
+                               (!d_records || !d_span_step || !d_chunk_state)))
+        return pc_fail(PC_EINVAL, "pc_microcanonical_sums: NULL buffer");
+    hipStream_t s = as_stream(stream);
+    const uint64_t S = (uint64_t)num_edges + 1u;
+    fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum1, 7u * S * 2u);
+    fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum2, 7u * S * 3u);
+    PC_LAUNCH_CHECK("fill_zero_kernel");
+    if (num_runs == 0)
+        return PC_OK;
+    if (num_edges == 0) {
+        step0_sums_kernel<<<1, 1, 0, s>>>(num_nodes, num_runs, S, d_sum1, d_sum2);
+        PC_LAUNCH_CHECK("step0_sums_kernel");
+        return PC_OK;
+    }
+    const uint32_t C = (num_edges + 63u) / 64u;
+    // enough blocks to fill 256 CUs several times over
+    uint32_t groups = (8192u + C - 1u) / C;
+    if (groups > num_runs)
+        groups = num_runs;
+    const uint32_t rpg = (num_runs + groups - 1u) / groups;
+    groups = (num_runs + rpg - 1u) / rpg;
+    const dim3 grid(C, groups);
+    if (num_nodes <= kLdsMaxNodes)
+        micro_sums_kernel<uint32_t, false><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, rpg, (const uint32_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, d_sum1, d_sum2);
+    else if (num_nodes <= kU32RecordMaxNodes)
+        micro_sums_kernel<uint32_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, rpg, (const uint32_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, d_sum1, d_sum2);
+    else
+        micro_sums_kernel<uint64_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, rpg, (const uint64_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, d_sum1, d_sum2);
+    PC_LAUNCH_CHECK("micro_sums_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges,
+                                uint32_t num_runs, const void *d_records,
+                                int64_t rec_stride, const uint32_t *d_span_step,
+                                const uint64_t *d_chunk_state,
+                                uint32_t *d_max_size, uint8_t *d_spanning,
+                                uint64_t *d_moments, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0)
+        return PC_OK;
+    if (num_nodes == 0 || num_nodes > kSumsMaxNodes)
+        return pc_fail(PC_EOVERFLOW, "pc_expand_traces: num_nodes outside 1..2^21");
+    if (!d_max_size || !d_spanning || !d_moments ||
+        (num_edges && (!d_records || !d_span_step || !d_chunk_state)))
+        return pc_fail(PC_EINVAL, "pc_expand_traces: NULL buffer");
+    hipStream_t s = as_stream(stream);
+    const uint64_t S = (uint64_t)num_edges + 1u;
+    if (num_edges == 0) {
+        expand_step0_kernel<<<(num_runs + 255u) / 256u, 256, 0, s>>>(
+            num_nodes, num_runs, S, d_max_size, d_spanning, d_moments);
+        PC_LAUNCH_CHECK("expand_step0_kernel");
+        return PC_OK;
+    }
+    const uint32_t C = (num_edges + 63u) / 64u;
+    const dim3 grid(C, num_runs);
+    if (num_nodes <= kLdsMaxNodes)
+        expand_kernel<uint32_t, false><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
+            d_span_step, d_chunk_state, d_max_size, d_spanning, d_moments);
+    else if (num_nodes <= kU32RecordMaxNodes)
+        expand_kernel<uint32_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
+            d_span_step, d_chunk_state, d_max_size, d_spanning, d_moments);
+    else
+        expand_kernel<uint64_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, (const uint64_t *)d_records, rec_stride,
+            d_span_step, d_chunk_state, d_max_size, d_spanning, d_moments);
+    PC_LAUNCH_CHECK("expand_kernel");
+    return PC_OK;
+}

@@ -1,0 +1,155 @@
+"""GPU parity of the sweep pipeline (permute -> sweep -> chunk_scan ->
+microcanonical sums / traces) against the oracle and the golden fixtures.
+Integer results must be bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from pypercolate_amd import _graph
+from pypercolate_amd._engine import BondEngine, host_sums
+from tests.helpers import grid_3x3_graph, spanning_2d_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine_for(nx_graph, spanning):
+    from pypercolate_amd.percolate import percolation_graph
+    pg = percolation_graph(nx_graph, spanning_cluster=spanning)
+    ga = _graph.from_percolation_graph(**pg)
+    return BondEngine(ga), oracle.percolation_graph_arrays(nx_graph, spanning)
+
+
+def _check_traces(eng, res, expect, runs=None):
+    mx, sp, mom = eng.host_traces(res)
+    rows = slice(None) if runs is None else runs
+    np.testing.assert_array_equal(mx, expect['max_cluster_size'][rows])
+    np.testing.assert_array_equal(sp, expect['has_spanning_cluster'][rows])
+    np.testing.assert_array_equal(mom, expect['moments'][rows])
+
+
+def _check_sums(eng, res, expect):
+    s1, s2 = host_sums(*eng.microcanonical_sums(res))
+    o1, o2 = oracle.microcanonical_sums(expect)
+    assert (s1 == o1).all(), 'sum of x differs'
+    assert (s2 == o2).all(), 'sum of x**2 differs'
+
+
+@pytest.mark.parametrize('spanning', [True, False])
+def test_kat_3x3(gpu, golden, spanning):
+    kat = golden('kat_3x3.json')
+    eng, g = _engine_for(grid_3x3_graph(spanning, kat), spanning)
+    perm = np.array([kat['permutation']])
+    res = eng.sweep(eng.upload_permutations(perm))
+    mx, sp, mom = eng.host_traces(res)
+    assert mx[0].tolist() == kat['max_cluster_size']
+    assert mom[0].tolist() == kat['moments']
+    if spanning:
+        assert sp[0].tolist() == kat['has_spanning_cluster']
+
+
+@pytest.mark.parametrize('length', [2, 3, 4, 6])
+def test_brute_force_fixtures(gpu, golden, length):
+    bf = golden('brute_force.npz')
+    eng, g = _engine_for(spanning_2d_grid(length), True)
+    res = eng.sweep(eng.upload_permutations(bf['L{}_perm'.format(length)]))
+    mx, sp, mom = eng.host_traces(res)
+    np.testing.assert_array_equal(mx, bf['L{}_max'.format(length)])
+    np.testing.assert_array_equal(mom, bf['L{}_moments'.format(length)])
+    np.testing.assert_array_equal(sp, bf['L{}_spanning'.format(length)])
+
+
+@pytest.mark.parametrize('spanning', [True, False])
+def test_config_L16_40_runs(gpu, spanning):
+    """spanning_2d_grid(16), 40 runs with RandomState(r) permutations."""
+    eng, g = _engine_for(spanning_2d_grid(16), spanning)
+    perms = oracle.randomstate_permutations(range(40), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms))
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+
+
+def test_closed_form_lattice_equals_networkx(gpu):
+    eng_nx, g = _engine_for(spanning_2d_grid(12), True)
+    eng_cf = BondEngine(_graph.square_lattice(12))
+    perms = oracle.randomstate_permutations(range(8), g['num_edges'])
+    a = eng_nx.host_traces(eng_nx.sweep(eng_nx.upload_permutations(perms)))
+    b = eng_cf.host_traces(eng_cf.sweep(eng_cf.upload_permutations(perms)))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_random_graph_no_spanning(gpu):
+    import networkx as nx
+    gnx = nx.gnm_random_graph(300, 900, seed=11)
+    eng, g = _engine_for(gnx, False)
+    perms = oracle.randomstate_permutations(range(70), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms))
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+
+
+def test_L128_64_runs(gpu):
+    """The L = 128 lattice (N = 16384, M = 32512), LDS state at its limit."""
+    eng = BondEngine(_graph.square_lattice(128))
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(128), True)
+    perms = oracle.randomstate_permutations(range(64), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms))
+    _check_sums(eng, res, expect)
+    _check_traces(eng, res, expect)
+
+
+@pytest.mark.parametrize('length', [130, 257])
+def test_global_state(gpu, length):
+    """N > 16384 keeps the union-find in HBM; N > 65536 uses 64-bit records."""
+    eng = BondEngine(_graph.square_lattice(length))
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(length), True)
+    perms = oracle.randomstate_permutations(range(6), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms))
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+
+
+@pytest.mark.parametrize('M', [1, 12, 480, 5000])
+def test_device_permutation_matches_oracle(gpu, M):
+    eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
+        [np.arange(M), np.arange(1, M + 1)], axis=1)))
+    d = eng.device_permutations(seed=1234, num_runs=3, first_run=5)
+    got = d.cpu().numpy()
+    for r in range(3):
+        np.testing.assert_array_equal(got[r], oracle.feistel_permutation(1234, 5 + r, M))
+
+
+def test_device_permutation_L128_is_permutation(gpu):
+    eng = BondEngine(_graph.square_lattice(128))
+    d = eng.device_permutations(seed=1234, num_runs=32)
+    srt = d.sort(dim=1).values.cpu().numpy()
+    np.testing.assert_array_equal(srt, np.broadcast_to(np.arange(eng.M), srt.shape))
+
+
+def test_full_size_properties(gpu):
+    """L = 128, 10^4 runs from device permutations: size-independent checks
+    of the exact sums (end state, identities between columns)."""
+    eng = BondEngine(_graph.square_lattice(128))
+    R = 10000
+    res = eng.sweep(eng.device_permutations(seed=1234, num_runs=R))
+    s1, s2 = host_sums(*eng.microcanonical_sums(res))
+    N, M = eng.N, eng.M
+    # step 0 and step M
+    assert s1[0, 0] == R and s1[0, M] == R * N
+    assert s1[1, 0] == 0 and s1[1, M] == R
+    assert all(s1[2 + k, M] == 0 for k in range(5))
+    assert all(s1[2 + k, 0] == R * (N - 1) for k in range(5))
+    # moments[1] = N - max for every run and step
+    assert (s1[3] == R * N - s1[0]).all()
+    # spanning is 0/1: sum of squares equals sum
+    assert (s2[1] == s1[1]).all()
+    # max cluster size and spanning count never decrease
+    assert (np.diff(s1[0].astype(float)) >= 0).all()
+    assert (np.diff(s1[1].astype(float)) >= 0).all()
+    # Cauchy-Schwarz: R * sum x^2 >= (sum x)^2
+    for c in range(7):
+        assert (R * s2[c] - s1[c] * s1[c] >= 0).all()

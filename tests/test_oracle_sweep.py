@@ -1,0 +1,153 @@
+"""Pins the CPU oracle of the sweep (oracle/nz_oracle.c) before it is used to
+check the GPU: the reference's own known-answer test, the tutorial traces,
+brute-force connected components and closed-form states."""
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests.helpers import grid_3x3_graph, spanning_1d_chain, spanning_2d_grid
+
+
+@pytest.mark.parametrize('spanning', [True, False])
+def test_kat_3x3(golden, spanning):
+    """percolate/test/test_percolate.py:132-192."""
+    kat = golden('kat_3x3.json')
+    g = oracle.percolation_graph_arrays(grid_3x3_graph(spanning, kat), spanning)
+    # the drawn permutation is numpy's legacy stream (global seed 42)
+    np.random.seed(kat['seed'])
+    perm = np.random.permutation(g['num_edges'])
+    assert perm.tolist() == kat['permutation']
+    tr = oracle.bond_runs(g, perm[None, :])
+    assert tr['max_cluster_size'][0].tolist() == kat['max_cluster_size']
+    assert tr['moments'][0].tolist() == kat['moments']
+    if spanning:
+        assert tr['has_spanning_cluster'][0].tolist() == kat['has_spanning_cluster']
+    else:
+        assert not tr['has_spanning_cluster'].any()
+    assert tr['edge'][0, 1:].tolist() == kat['permutation']
+
+
+def _tutorial_graph(trace):
+    if trace['graph'] == 'chain':
+        return spanning_1d_chain(trace['length'])
+    return spanning_2d_grid(trace['length'])
+
+
+def _label(x):
+    return tuple(_label(y) for y in x) if isinstance(x, list) else x
+
+
+@pytest.mark.parametrize('which', [0, 1])
+def test_tutorial_traces(golden, which):
+    """docs/tutorial-bond-square-lattice.ipynb: every printed state, with the
+    permutation recovered from the printed edges (including the tie of two
+    largest clusters at n = 5 of the grid)."""
+    trace = golden('tutorial_traces.json')['traces'][which]
+    g = oracle.percolation_graph_arrays(_tutorial_graph(trace), True)
+    index = {frozenset(e): i for i, e in enumerate(g['edge_list'])}
+    states = trace['states']
+    perm = np.array([index[frozenset(_label(s['edge']))] for s in states[1:]])
+    assert sorted(perm.tolist()) == list(range(g['num_edges']))
+    tr = oracle.bond_runs(g, perm[None, :])
+    for n, st in enumerate(states):
+        assert st['n'] == n
+        assert st['N'] == g['num_nodes'] and st['M'] == g['num_edges']
+        assert tr['max_cluster_size'][0, n] == st['max_cluster_size']
+        assert tr['moments'][0, n].tolist() == st['moments']
+        assert bool(tr['has_spanning_cluster'][0, n]) == st['has_spanning_cluster']
+
+
+@pytest.mark.parametrize('length', [2, 3, 4, 6])
+def test_brute_force_fixtures(golden, length):
+    """Brute-force connected components (test_hpc.py:250-292 strategy)."""
+    bf = golden('brute_force.npz')
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(length), True)
+    perm = bf['L{}_perm'.format(length)]
+    tr = oracle.bond_runs(g, perm)
+    np.testing.assert_array_equal(tr['max_cluster_size'], bf['L{}_max'.format(length)])
+    np.testing.assert_array_equal(tr['moments'], bf['L{}_moments'.format(length)])
+    np.testing.assert_array_equal(tr['has_spanning_cluster'],
+                                  bf['L{}_spanning'.format(length)])
+
+
+def _brute(perc_nodes, edge_list, perm, n):
+    sub = nx.Graph()
+    sub.add_nodes_from(perc_nodes)
+    sub.add_edges_from(edge_list[perm[i]] for i in range(n))
+    return sorted((len(c) for c in nx.connected_components(sub)), reverse=True)
+
+
+@pytest.mark.parametrize('seed', [0, 1, 7, 4294967295])
+def test_brute_force_random_graph(seed):
+    """Arbitrary graphs (not lattices): random graph without spanning."""
+    rng = np.random.RandomState(seed % 1000)
+    g_nx = nx.gnm_random_graph(30, 70, seed=int(seed % 1000))
+    g = oracle.percolation_graph_arrays(g_nx, False)
+    perm = np.random.RandomState(seed).permutation(g['num_edges'])
+    tr = oracle.bond_runs(g, perm[None, :])
+    for n in range(0, g['num_edges'] + 1, 7):
+        sizes = _brute(g['nodes'], g['edge_list'], perm, n)
+        assert tr['max_cluster_size'][0, n] == sizes[0]
+        for k in range(5):
+            assert tr['moments'][0, n, k] == sum(s ** k for s in sizes[1:])
+    del rng
+
+
+def test_closed_forms():
+    """n = 0 (hpc.py:213-218) and the connected end state; monotonicity."""
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(8), True)
+    perms = oracle.randomstate_permutations(range(5), g['num_edges'])
+    tr = oracle.bond_runs(g, perms)
+    N = g['num_nodes']
+    assert (tr['max_cluster_size'][:, 0] == 1).all()
+    assert (tr['moments'][:, 0] == N - 1).all()
+    assert not tr['has_spanning_cluster'][:, 0].any()
+    assert (tr['max_cluster_size'][:, -1] == N).all()
+    assert (tr['moments'][:, -1] == 0).all()
+    assert tr['has_spanning_cluster'][:, -1].all()
+    assert (np.diff(tr['max_cluster_size'].astype(np.int64), axis=1) >= 0).all()
+    assert (np.diff(tr['moments'][:, :, 0].astype(np.int64), axis=1) <= 0).all()
+    # moments[1] = N - max; spanning is monotone
+    np.testing.assert_array_equal(tr['moments'][:, :, 1],
+                                  N - tr['max_cluster_size'].astype(np.uint64))
+    assert (np.diff(tr['has_spanning_cluster'].astype(int), axis=1) >= 0).all()
+
+
+def test_microcanonical_sums_exact():
+    """The C sums equal Python-int sums of the traces."""
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(5), True)
+    perms = oracle.randomstate_permutations(range(9), g['num_edges'])
+    tr = oracle.bond_runs(g, perms)
+    s1, s2 = oracle.microcanonical_sums(tr)
+    cols = [tr['max_cluster_size'], tr['has_spanning_cluster'].astype(np.uint64)] + \
+        [tr['moments'][:, :, k] for k in range(5)]
+    for c, x in enumerate(cols):
+        xo = x.astype(object)
+        assert (s1[c] == xo.sum(axis=0)).all()
+        assert (s2[c] == (xo * xo).sum(axis=0)).all()
+
+
+def test_moments_wrap_mod_2_64():
+    """uint64 moments wrap like the reference's numpy uint64 (hpc.py:286-305):
+    a chain of 70000 nodes merged in order has 65535**4 > 2**64 intermediates."""
+    big = 1 << 16
+    n = 2 * big + 10
+    g = {'num_nodes': n, 'num_edges': n - 1, 'spanning': False,
+         'edges': np.stack([np.arange(n - 1), np.arange(1, n)], axis=1),
+         'aux_side': np.zeros(0, np.int32), 'aux_edges': np.zeros((0, 2), np.int64),
+         'aux_edge_side': np.zeros(0, np.int32)}
+    # grow cluster A = nodes 0..2^16-1, then cluster B = nodes 2^16..2^17-1
+    # (B ties A and is not the largest), then the remaining bonds
+    first = np.arange(big - 1)
+    second = np.arange(big, 2 * big - 1)
+    rest = np.setdiff1d(np.arange(n - 1), np.concatenate([first, second]))
+    perm = np.concatenate([first, second, rest])
+    tr = oracle.bond_runs(g, perm[None, :])
+    s = 2 * (big - 1)
+    singles = n - 2 * big
+    expect = np.uint64(singles) + np.uint64(big) ** np.arange(5, dtype=np.uint64)
+    expect[0] = singles + 1
+    assert tr['max_cluster_size'][0, s] == big
+    assert tr['moments'][0, s].tolist() == expect.tolist()
+    assert int(expect[4]) == singles  # 2**64 wrapped to 0
