@@ -1,0 +1,209 @@
+"""Throughput of the Newman-Ziff bond-percolation hot path on one MI355X.
+
+Workload (default): bond percolation on the L = 128 open square lattice
+(spanning_2d_grid(128): N = 16384 nodes, M = 32512 bonds, top/bottom
+spanning sides), R = 10^4 runs per step.  One step is one pass of the hot
+path over the batch: device permutations (Philox seed 1234) -> union-find
+sweep -> per-run chunk states -> exact microcanonical sums over runs for
+every step n = 0..M.  Inputs (the lattice) are resident in HBM before the
+timed region.  The metric is bond additions per second = R * M / time.
+
+Output: one JSON line.  ``roofline`` describes the sweep kernel (the
+dominant kernel), timed with HIP events on the stream it runs on;
+``cpu_baseline`` times the CPU oracle (the C restatement of
+percolate/hpc.py:194-307 plus the exact per-step sums) on a bounded sample
+of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ('bond additions/sec at L=128 + canonical-avg speedup FP32-MFMA vs '
+          'FP64 @1e-9')
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per bond addition in the LDS-state sweep: the 4-byte
+# permutation read from HBM plus six 2-byte LDS accesses (two parent reads
+# for the finds, two root-size reads, one parent and one size write)
+ALG_BYTES_PER_ADD_LDS = 16.0
+ALG_BYTES_PER_ADD_HBM = 28.0
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split('\n\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--length', type=int, default=128)
+    ap.add_argument('--runs', type=int, default=10000)
+    ap.add_argument('--seed', type=int, default=1234)
+    ap.add_argument('--cpu-sample-runs', type=int, default=2048,
+                    help='runs in the CPU-oracle sample (0 skips it)')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='also run a 1 GiB device copy (for PMC scaling)')
+    return ap.parse_args(argv)
+
+
+def cpu_baseline(length, sample_runs, seed):
+    """The oracle on a bounded sample: sweep + exact per-step sums."""
+    import numpy as np
+    from oracle import oracle
+    from pypercolate_amd import _graph
+
+    ga = _graph.square_lattice(length)
+    L = length
+    N, M = ga.num_nodes, ga.num_edges
+    g = {'num_nodes': N, 'num_edges': M, 'spanning': True,
+         'edges': ga.edges.astype(np.int64),
+         'aux_side': np.array([0] * L + [1] * L, dtype=np.int32),
+         'aux_edges': np.array([(i, N + i) for i in range(L)] +
+                               [(N - L + i, N + L + i) for i in range(L)],
+                               dtype=np.int64),
+         'aux_edge_side': np.array([0] * L + [1] * L, dtype=np.int32)}
+    threads = int(os.environ.get('OMP_NUM_THREADS', 0)) or os.cpu_count() or 1
+    batch = 128
+    perms = oracle.randomstate_permutations(range(seed, seed + batch), M)
+    t = 0.0
+    done = 0
+    while done < sample_runs:
+        t0 = time.perf_counter()
+        tr = oracle.bond_runs(g, perms, threads=threads)
+        oracle.microcanonical_sums(tr)
+        t += time.perf_counter() - t0
+        done += batch
+        del tr
+    return {
+        'value': done * M / t,
+        'unit': 'bond additions/s',
+        'cores': threads,
+        'kind': 'port',
+        'sample': ('{} runs of L={} (host RandomState permutations, not timed), '
+                   'C oracle sweep with uint64 traces + exact per-step sums, '
+                   'OpenMP over runs; {:.2f} s').format(done, L, t),
+    }
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU '
+              'path (runs would shard by run; see DESIGN.md)', file=sys.stderr)
+        return 2
+
+    import torch
+    from pypercolate_amd import _graph, _native as nat
+    from pypercolate_amd._engine import BondEngine
+
+    torch.cuda.set_device(0)
+    eng = BondEngine(_graph.square_lattice(args.length))
+    R, M, N, S = args.runs, eng.M, eng.N, eng.S
+    dev = eng.device
+    C = (M + nat.PC_CHUNK - 1) // nat.PC_CHUNK
+    perm = torch.empty((R, M), dtype=torch.int32, device=dev)
+    records = torch.empty((R, M), dtype=eng.rec_torch, device=dev)
+    span = torch.empty((R,), dtype=torch.int32, device=dev)
+    state = torch.empty((R, C, nat.PC_STATE_WORDS), dtype=torch.int64, device=dev)
+    sum1 = torch.empty((7, S, 2), dtype=torch.int64, device=dev)
+    sum2 = torch.empty((7, S, 3), dtype=torch.int64, device=dev)
+    wb = eng.workspace_bytes(R)
+    workspace = torch.empty((wb,), dtype=torch.uint8, device=dev) if wb else None
+
+    sweep_events = [(torch.cuda.Event(enable_timing=True),
+                     torch.cuda.Event(enable_timing=True))
+                    for _ in range(args.steps)]
+
+    def step(events=None):
+        eng.device_permutations(args.seed, R, out=perm)
+        if events:
+            events[0].record()
+        res = eng.sweep(perm, records=records, span_step=span,
+                        workspace=workspace)
+        if events:
+            events[1].record()
+        res.chunk_state = eng.chunk_scan(res, out=state)
+        eng.microcanonical_sums(res, sum1=sum1, sum2=sum2)
+        return res
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record()
+    for k in range(args.steps):
+        step(sweep_events[k])
+    stop.record()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    total_ms = start.elapsed_time(stop)
+    ms_per_step = total_ms / args.steps
+    sweep_avg_ms = sum(a.elapsed_time(b) for a, b in sweep_events) / args.steps
+
+    # sanity on the last step's exact sums (not timed): at n = M every run
+    # is one cluster of N nodes and spans
+    s1 = sum1.cpu().numpy().view('uint64')
+    ok = int(s1[0, M, 0]) == R * N and int(s1[1, M, 0]) == R
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    adds = float(R) * M
+    value = adds / (ms_per_step / 1e3)
+    alg_bytes = ALG_BYTES_PER_ADD_LDS if N <= 16384 else ALG_BYTES_PER_ADD_HBM
+    achieved = alg_bytes * adds / (sweep_avg_ms / 1e3) / 1e9
+    out = {
+        'metric': METRIC,
+        'value': value,
+        'unit': 'bond additions/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'u16/u32/u64/u128',
+        'data': 'synthetic: Philox-keyed device permutations, seed {}'.format(
+            args.seed),
+        'config': {
+            'workload': ('bond percolation, L={} square lattice '
+                         '(N={}, M={}), {} runs per step: permute -> sweep '
+                         '-> chunk states -> exact microcanonical sums').format(
+                             args.length, N, M, R),
+            'length': args.length, 'runs': R, 'nodes': N, 'edges': M,
+            'parallelism': 'single GPU, one run per wavefront',
+        },
+        'roofline': {
+            'kernel': 'sweep_lds_kernel' if N <= 16384 else 'sweep_global_kernel',
+            'bound': 'hbm',
+            'achieved': achieved,
+            'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s',
+            'frac': achieved / HBM_PEAK_GBS,
+            'traffic': None,
+            'alg_bytes_per_add': alg_bytes,
+            'kernel_ms': sweep_avg_ms,
+        },
+        'sums_check': ok,
+        'wall_s': t_wall,
+    }
+    if args.cpu_sample_runs > 0:
+        out['cpu_baseline'] = cpu_baseline(args.length, args.cpu_sample_runs,
+                                           args.seed)
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == '__main__':
+    sys.exit(main())

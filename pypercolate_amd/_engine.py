@@ -32,6 +32,10 @@ class SweepResult(object):
             self._chunk_state = self.engine.chunk_scan(self)
         return self._chunk_state
 
+    @chunk_state.setter
+    def chunk_state(self, value):
+        self._chunk_state = value
+
 
 class BondEngine(object):
     """One percolation graph resident on the GPU."""
@@ -86,7 +90,7 @@ class BondEngine(object):
 
     def sweep(self, d_perm, records=None, span_step=None, workspace=None):
         torch = self.torch
-        R = int(d_perm.shape[0]) if self.M else int(d_perm.shape[0])
+        R = int(d_perm.shape[0])
         if records is None:
             records = torch.empty((R, max(self.M, 1)), dtype=self.rec_torch,
                                   device=self.device)
@@ -103,11 +107,12 @@ class BondEngine(object):
             nat.ptr(workspace), wb, st), 'pc_sweep')
         return SweepResult(self, R, records, span_step, d_perm)
 
-    def chunk_scan(self, res):
+    def chunk_scan(self, res, out=None):
         torch = self.torch
         C = max((self.M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
-        out = torch.empty((res.num_runs, C, nat.PC_STATE_WORDS),
-                          dtype=torch.int64, device=self.device)
+        if out is None:
+            out = torch.empty((res.num_runs, C, nat.PC_STATE_WORDS),
+                              dtype=torch.int64, device=self.device)
         st = nat.stream_handle(torch)
         nat.check(self.lib.pc_chunk_scan(
             self.N, self.M, res.num_runs, nat.ptr(res.records),
@@ -115,14 +120,16 @@ class BondEngine(object):
         return out
 
     # -- outputs ------------------------------------------------------------
-    def microcanonical_sums(self, res):
+    def microcanonical_sums(self, res, sum1=None, sum2=None):
         """Exact per-step sums over runs: device tensors sum1 (7, S, 2) and
         sum2 (7, S, 3) of uint64 limbs (stored as int64)."""
         torch = self.torch
-        sum1 = torch.empty((nat.PC_COLUMNS, self.S, 2), dtype=torch.int64,
-                           device=self.device)
-        sum2 = torch.empty((nat.PC_COLUMNS, self.S, 3), dtype=torch.int64,
-                           device=self.device)
+        if sum1 is None:
+            sum1 = torch.empty((nat.PC_COLUMNS, self.S, 2), dtype=torch.int64,
+                               device=self.device)
+        if sum2 is None:
+            sum2 = torch.empty((nat.PC_COLUMNS, self.S, 3), dtype=torch.int64,
+                               device=self.device)
         st = nat.stream_handle(torch)
         nat.check(self.lib.pc_microcanonical_sums(
             self.N, self.M, res.num_runs, nat.ptr(res.records),

@@ -273,11 +273,13 @@ struct LdsState {
     __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { code[i] = (uint16_t)v; }
     __device__ __forceinline__ bool is_root(uint32_t v) const { return v >= N; }
     __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return N * (1u + f) + size - 1u; }
+    // LDS operations of one wavefront complete in order
+    __device__ __forceinline__ void sync() const {}
     __device__ __forceinline__ void decode(uint32_t v, uint32_t &size, uint32_t &f) const
     {
         const uint32_t j = v - N;
-        f = (uint32_t)(j >= N) + (uint32_t)(j >= 2u * N);
-        size = j - f * N + 1u;
+        f = j >= 2u * N ? 2u : (j >= N ? 1u : 0u);
+        size = j - (j >= 2u * N ? 2u * N : (j >= N ? N : 0u)) + 1u;
     }
 };
 
@@ -289,6 +291,8 @@ struct GlobalState {
     __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { code[i] = v; }
     __device__ __forceinline__ bool is_root(uint32_t v) const { return (v >> 31) != 0u; }
     __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return 0x80000000u | (f << 29) | size; }
+    // stores of some lanes must be visible to loads of others
+    __device__ __forceinline__ void sync() const { __threadfence_block(); }
     __device__ __forceinline__ void decode(uint32_t v, uint32_t &size, uint32_t &f) const
     {
         size = v & 0x1FFFFFFFu;
@@ -354,18 +358,52 @@ __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
     }
 }
 
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Puts a uniform value into one lane (a compare and a select, no branch).
+template <typename T>
+__device__ __forceinline__ T write_lane(T v, uint32_t lane, T old)
+{
+    return threadIdx.x == lane ? v : old;
+}
+
+// Conflict table for the parallel merge rounds: CLAIM_SLOTS words in LDS,
+// indexed by a multiplicative hash of a root.
+#define CLAIM_BITS 10
+#define CLAIM_SLOTS (1u << CLAIM_BITS)
+
+__device__ __forceinline__ uint32_t claim_slot(uint32_t root)
+{
+    return (root * 2654435761u) >> (32 - CLAIM_BITS);
+}
+
 // One run on one wavefront.  Bonds are taken 64 at a time (lane = step):
-//  1. every lane finds the roots of its bond's endpoints in parallel;
-//  2. lanes whose roots differ are candidate merges; they are resolved in
-//     step order on scalar registers (readlane), and after each merge every
-//     lane relabels its cached roots, so later candidates see the merge;
-//  3. the new parent pointers and root codes are written back in parallel.
+//  1. every lane finds the roots of its bond's endpoints in parallel (path
+//     halving); lanes whose roots differ are candidate merges;
+//  2. parallel rounds: each candidate claims the table slots of its two
+//     roots with an atomic max of (round tag + 63 - lane), so each slot ends
+//     up held by the earliest candidate touching it.  A candidate holding
+//     both of its slots shares no root with any earlier pending candidate:
+//     no earlier merge can change its clusters, so it merges now, in
+//     parallel with the other such candidates (their roots are distinct).
+//     The others re-find their roots and go to the next round.  Hash
+//     collisions only make the test conservative;
+//  3. the candidates left after `rounds` rounds are resolved in step order
+//     with uniform control flow: roots re-read from the state (one address
+//     for the whole wave, made uniform with readfirstlane) and followed
+//     while earlier merges moved them, the union decided on scalar
+//     registers, lane 0 writing it back, the record placed with a lane
+//     select.
 // Per-step output is one merge record; the spanning step is kept scalar.
 template <class St, typename RecT>
 __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           const int2 *__restrict__ edges,
                                           const uint8_t *__restrict__ sides,
-                                          bool span_first,
+                                          bool span_first, uint32_t rounds,
+                                          uint32_t *__restrict__ claims,
                                           const int32_t *__restrict__ perm,
                                           RecT *__restrict__ rec_out,
                                           uint32_t *__restrict__ span_out)
@@ -373,9 +411,12 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < N; i += 64u)
         st.set(i, st.root(1u, sides ? (uint32_t)sides[i] : 0u));
+    for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
+        claims[i] = 0u;
     __syncthreads();
 
     uint32_t span_at = PC_NO_SPAN;
+    uint32_t tag = 0;
 
     // software pipeline: the permutation two batches ahead and the edge
     // endpoints one batch ahead are in flight while a batch is resolved
@@ -393,21 +434,67 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 
         uint32_t ru = (uint32_t)uv.x, rv = (uint32_t)uv.y, cu, cv;
         find2(st, ru, cu, rv, cv);
-        const bool cand = active && ru != rv;
+        bool cand = active && ru != rv;
         uint64_t pending = __ballot(cand);
-
         RecT rec = 0;
-        uint32_t small_l = 0, big_l = 0;
-        bool merged = false;
+
+        for (uint32_t k = 0; k < rounds && pending; ++k) {
+            tag += 64u;
+            const uint32_t mine = tag + 63u - lane;
+            const uint32_t su = claim_slot(ru), sv = claim_slot(rv);
+            if (cand) {
+                atomicMax(&claims[su], mine);
+                atomicMax(&claims[sv], mine);
+            }
+            const bool go = cand && claims[su] == mine && claims[sv] == mine;
+            if (go) {
+                uint32_t sa, fa, sb, fb;
+                st.decode(cu, sa, fa);
+                st.decode(cv, sb, fb);
+                const bool a_big = sa >= sb;
+                const uint32_t big = a_big ? ru : rv, small = a_big ? rv : ru;
+                uint32_t f = fa | fb;
+                st.set(small, big);
+                st.set(big, st.root(sa + sb, f == 3u ? 0u : f));
+                rec = pack_record<RecT>(sa, sb);
+            }
+            {
+                uint32_t fa, fb, sz;
+                st.decode(cu, sz, fa);
+                st.decode(cv, sz, fb);
+                const uint64_t spans = __ballot(go && (span_first || (fa | fb) == 3u));
+                if (spans)
+                    span_at = min(span_at, base + (uint32_t)__builtin_ctzll(spans) + 1u);
+            }
+            cand = cand && !go;
+            st.sync();
+            if (__ballot(cand)) {
+                // lanes still pending follow their roots past this round's merges
+                if (cand)
+                    find2(st, ru, cu, rv, cv);
+                cand = cand && ru != rv;
+            }
+            pending = __ballot(cand);
+        }
+
         while (pending) {
             const uint32_t i = (uint32_t)__builtin_ctzll(pending);
             pending &= pending - 1u;
-            const uint32_t a = read_lane(ru, i), b = read_lane(rv, i);
+            uint32_t a = read_lane(ru, i), b = read_lane(rv, i);
+            uint32_t ca = uniform(st.get(a)), cb = uniform(st.get(b));
+            while (!st.is_root(ca)) {
+                a = ca;
+                ca = uniform(st.get(a));
+            }
+            while (!st.is_root(cb)) {
+                b = cb;
+                cb = uniform(st.get(b));
+            }
             if (a == b)
                 continue; // joined by an earlier bond of this batch
             uint32_t sa, fa, sb, fb;
-            st.decode(read_lane(cu, i), sa, fa);
-            st.decode(read_lane(cv, i), sb, fb);
+            st.decode(ca, sa, fa);
+            st.decode(cb, sb, fb);
             const bool a_big = sa >= sb;
             const uint32_t big = a_big ? a : b, small = a_big ? b : a;
             uint32_t f = fa | fb;
@@ -415,26 +502,11 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 span_at = min(span_at, base + i + 1u);
             if (f == 3u)
                 f = 0u;
-            const uint32_t nc = st.root(sa + sb, f);
-            ru = (ru == small) ? big : ru;
-            cu = (ru == big) ? nc : cu;
-            rv = (rv == small) ? big : rv;
-            cv = (rv == big) ? nc : cv;
-            if (lane == i) {
-                rec = pack_record<RecT>(sa, sb);
-                small_l = small;
-                big_l = big;
-                merged = true;
+            if (lane == 0) {
+                st.set(small, big);
+                st.set(big, st.root(sa + sb, f));
             }
-        }
-        // pointer writes go to the roots merged away in this batch, root
-        // writes to the surviving roots: disjoint addresses, and lanes that
-        // share a root write the same value
-        if (merged)
-            st.set(small_l, big_l);
-        if (cand) {
-            st.set(ru, cu);
-            st.set(rv, cv);
+            rec = write_lane(pack_record<RecT>(sa, sb), i, rec);
         }
         if (active)
             rec_out[n] = rec;
@@ -450,12 +522,13 @@ __global__ __launch_bounds__(64) void sweep_lds_kernel(
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
-    uint32_t *__restrict__ span_step)
+    uint32_t *__restrict__ span_step, uint32_t rounds)
 {
-    extern __shared__ uint16_t lds_code[];
+    // claim table first (word aligned), then the 16-bit codes
+    extern __shared__ uint32_t lds_words[];
     const uint32_t r = blockIdx.x;
-    const LdsState st{lds_code, N};
-    sweep_run(st, N, M, edges, sides, span_first != 0,
+    const LdsState st{(uint16_t *)(lds_words + CLAIM_SLOTS), N};
+    sweep_run(st, N, M, edges, sides, span_first != 0, rounds, lds_words,
               perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
               span_step + r);
 }
@@ -466,11 +539,13 @@ __global__ __launch_bounds__(64) void sweep_global_kernel(
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
-    uint32_t *__restrict__ span_step, uint32_t *__restrict__ workspace)
+    uint32_t *__restrict__ span_step, uint32_t *__restrict__ workspace,
+    uint32_t rounds)
 {
+    __shared__ uint32_t claims[CLAIM_SLOTS];
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
-        sweep_run(st, N, M, edges, sides, span_first != 0,
+        sweep_run(st, N, M, edges, sides, span_first != 0, rounds, claims,
                   perm + (int64_t)r * perm_stride,
                   records + (int64_t)r * rec_stride, span_step + r);
         __syncthreads();
@@ -787,6 +862,12 @@ static const uint32_t kGlobalSlots = 4096u;
 
 static inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 
+// parallel merge rounds per batch before the in-order remainder
+#ifndef PC_MERGE_ROUNDS
+#define PC_MERGE_ROUNDS 2
+#endif
+static uint32_t g_merge_rounds = PC_MERGE_ROUNDS;
+
 extern "C" int pc_abi_version(void) { return 1; }
 
 extern "C" const char *pc_last_error(void) { return g_last_error; }
@@ -862,11 +943,12 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
     if (num_nodes <= kLdsMaxNodes) {
-        const size_t lds = ((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u;
+        const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
+                           (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
         sweep_lds_kernel<uint32_t><<<num_runs, 64, lds, s>>>(
             num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
             d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
-            d_span_step);
+            d_span_step, g_merge_rounds);
         PC_LAUNCH_CHECK("sweep_lds_kernel");
         return PC_OK;
     }
@@ -886,12 +968,12 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         sweep_global_kernel<uint32_t><<<(uint32_t)slots, 64, 0, s>>>(
             num_nodes, num_edges, num_runs, edges, d_node_sides,
             span_on_first_merge, d_perm, perm_stride, (uint32_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace);
+            rec_stride, d_span_step, (uint32_t *)d_workspace, g_merge_rounds);
     } else {
         sweep_global_kernel<uint64_t><<<(uint32_t)slots, 64, 0, s>>>(
             num_nodes, num_edges, num_runs, edges, d_node_sides,
             span_on_first_merge, d_perm, perm_stride, (uint64_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace);
+            rec_stride, d_span_step, (uint32_t *)d_workspace, g_merge_rounds);
     }
     PC_LAUNCH_CHECK("sweep_global_kernel");
     return PC_OK;
