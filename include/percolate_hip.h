@@ -121,6 +121,50 @@ int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
                      uint32_t *d_max_size, uint8_t *d_spanning,
                      uint64_t *d_moments, void *stream);
 
+/* Precisions of pc_canonical_contract. */
+#define PC_PREC_FP64 0      /* FP64 FMA on the vector ALUs                 */
+
+/*
+ * Convolution window of the binomial weights B(n, k; p): k in
+ * [*start, *start + *len), the k with |k - n p| <= 12 sigma + 48 (each tail
+ * outside holds at most exp(-72) of the mass).  Host-only, no GPU work.
+ */
+int pc_binomial_window(uint32_t n, double p, int64_t *start, int64_t *len);
+
+/*
+ * Binomial weights B(n, k; p) for num_p values p = d_ps[i] over the windows
+ * (d_start[i], d_len[i]): d_w[i * kmax + j] = B(n, d_start[i] + j; p),
+ * normalised to sum 1 over the window, 0 for j >= d_len[i].  Replaces the
+ * ratio recursion of percolate/percolate.py:1067-1109 (_binomial_pmf) with
+ * Loader's saddle-point form in FP64.
+ */
+int pc_binomial_weights(uint32_t n, uint32_t num_p, const double *d_ps,
+                        const int64_t *d_start, const int64_t *d_len,
+                        int64_t kmax, double *d_w, void *stream);
+
+/*
+ * Canonical averages, the convolution of percolate/percolate.py:1196-1221
+ * and percolate/hpc.py:496-512: d_out[i * cols + c] =
+ * sum_j d_w[i * kmax + j] * d_x[(d_start[i] + j) * x_stride + c] for
+ * c < cols <= 32, at the given precision (PC_PREC_*).
+ */
+int pc_canonical_contract(uint32_t num_steps, uint32_t cols, const double *d_x,
+                          int64_t x_stride, uint32_t num_p,
+                          const int64_t *d_start, const int64_t *d_len,
+                          const double *d_w, int64_t kmax, double *d_out,
+                          int32_t precision, void *stream);
+
+/*
+ * Mean and sample standard deviation (ddof = 1) over num_runs runs of each
+ * column and step, from pc_microcanonical_sums: d_mean[k * S + s],
+ * d_std[k * S + s] (FP64; the variance numerator is formed exactly).  The
+ * statistics of percolate/percolate.py:575-705 before the confidence
+ * intervals.
+ */
+int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
+                              const uint64_t *d_sum1, const uint64_t *d_sum2,
+                              double *d_mean, double *d_std, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -172,3 +172,102 @@ def host_sums(sum1, sum2):
     """Device sum tensors -> exact Python-int object arrays (7, S)."""
     return (limbs_to_int(sum1.cpu().numpy()),
             limbs_to_int(sum2.cpu().numpy()))
+
+
+class Weights(object):
+    """Windowed binomial weights for a set of p on the device."""
+
+    def __init__(self, n, ps, start, length, kmax, w):
+        self.n = n
+        self.ps = ps            # host float64 (P,)
+        self.start = start      # device int64 (P,)
+        self.length = length    # device int64 (P,)
+        self.host_start = None
+        self.host_length = None
+        self.kmax = kmax
+        self.w = w              # device float64 (P, kmax)
+
+
+def binomial_windows(n, ps):
+    """Host windows (start, length) of the weights for every p."""
+    lib = nat.load_library()
+    ps = np.atleast_1d(np.asarray(ps, dtype=np.float64))
+    start = np.empty(ps.size, dtype=np.int64)
+    length = np.empty(ps.size, dtype=np.int64)
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    for i, p in enumerate(ps):
+        nat.check(lib.pc_binomial_window(int(n), float(p), ctypes.byref(a),
+                                         ctypes.byref(b)), 'pc_binomial_window')
+        start[i], length[i] = a.value, b.value
+    return start, length
+
+
+def binomial_weights(n, ps):
+    """Device weights B(n, k; p), windowed and normalised, for every p."""
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    ps = np.atleast_1d(np.asarray(ps, dtype=np.float64))
+    start, length = binomial_windows(n, ps)
+    kmax = int(length.max()) if ps.size else 1
+    dev = torch.device('cuda', torch.cuda.current_device())
+    d_ps = torch.from_numpy(ps.copy()).to(dev)
+    d_start = torch.from_numpy(start).to(dev)
+    d_len = torch.from_numpy(length).to(dev)
+    w = torch.empty((max(ps.size, 1), kmax), dtype=torch.float64, device=dev)
+    nat.check(lib.pc_binomial_weights(int(n), ps.size, nat.ptr(d_ps),
+                                      nat.ptr(d_start), nat.ptr(d_len), kmax,
+                                      nat.ptr(w), nat.stream_handle(torch)),
+              'pc_binomial_weights')
+    wt = Weights(n, ps, d_start, d_len, kmax, w)
+    wt.host_start, wt.host_length = start, length
+    return wt
+
+
+def dense_weights(factors):
+    """Caller-supplied convolution factors (length S) as a one-row Weights."""
+    torch = nat.torch_cuda()
+    f = np.ascontiguousarray(factors, dtype=np.float64).reshape(-1)
+    dev = torch.device('cuda', torch.cuda.current_device())
+    wt = Weights(f.size - 1, None,
+                 torch.zeros(1, dtype=torch.int64, device=dev),
+                 torch.full((1,), f.size, dtype=torch.int64, device=dev),
+                 max(f.size, 1), torch.from_numpy(f.copy()).to(dev).reshape(1, -1))
+    wt.host_start = np.zeros(1, dtype=np.int64)
+    wt.host_length = np.array([f.size], dtype=np.int64)
+    return wt
+
+
+def contract(x, weights, precision=nat.PC_PREC_FP64):
+    """Canonical convolution of the columns of x (S, C) float64 device
+    tensor with every weight row: (P, C) float64 device tensor."""
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    x = x.contiguous()
+    S, C = int(x.shape[0]), int(x.shape[1])
+    P = int(weights.w.shape[0]) if weights.ps is None else len(weights.ps)
+    out = torch.empty((P, C), dtype=torch.float64, device=x.device)
+    if P and S:
+        if int(weights.host_start.min()) < 0 or int(
+                (weights.host_start + weights.host_length).max()) > S:
+            raise ValueError('convolution window outside the {} steps'.format(S))
+        nat.check(lib.pc_canonical_contract(
+            S, C, nat.ptr(x), C, P, nat.ptr(weights.start),
+            nat.ptr(weights.length), nat.ptr(weights.w), weights.kmax,
+            nat.ptr(out), int(precision), nat.stream_handle(torch)),
+            'pc_canonical_contract')
+    return out
+
+
+def micro_moments(sum1, sum2, num_edges, num_runs):
+    """Mean and sample std (ddof=1) per column and step: (7, S) float64."""
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    S = num_edges + 1
+    mean = torch.empty((nat.PC_COLUMNS, S), dtype=torch.float64,
+                       device=sum1.device)
+    std = torch.empty_like(mean)
+    nat.check(lib.pc_microcanonical_moments(
+        num_edges, num_runs, nat.ptr(sum1), nat.ptr(sum2), nat.ptr(mean),
+        nat.ptr(std), nat.stream_handle(torch)), 'pc_microcanonical_moments')
+    return mean, std
+

@@ -45,7 +45,19 @@ SIGNATURES = {
                                               _vp, _vp, _vp, _vp]),
     'pc_expand_traces': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp,
                                         _vp, _vp, _vp, _vp]),
+    'pc_binomial_window': (ctypes.c_int, [_u32, ctypes.c_double,
+                                          ctypes.POINTER(_i64),
+                                          ctypes.POINTER(_i64)]),
+    'pc_binomial_weights': (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _i64,
+                                           _vp, _vp]),
+    'pc_canonical_contract': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32, _vp,
+                                             _vp, _vp, _i64, _vp, _i32, _vp]),
+    'pc_microcanonical_moments': (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp,
+                                                 _vp, _vp]),
 }
+
+# precisions of pc_canonical_contract
+PC_PREC_FP64 = 0
 
 _lib = None
 

@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "percolate_hip.h"
+#include "stirlerr_table.h"
 
 typedef unsigned __int128 u128;
 
@@ -851,6 +852,256 @@ __global__ __launch_bounds__(64) void expand_kernel(
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Binomial weights B(n, k; p) = C(n, k) p^k (1-p)^(n-k), the convolution
+// factors of percolate/percolate.py:1067-1109, evaluated per element in FP64
+// with Loader's saddle-point form (C. Loader, "Fast and Accurate Computation
+// of Binomial Probabilities", 2000):
+//   ln B = stirlerr(n) - stirlerr(k) - stirlerr(n-k) - bd0(k, np)
+//          - bd0(n-k, nq) - ln(2 pi k (n-k) / n) / 2
+// with stirlerr(m) = ln m! - ln(sqrt(2 pi m) (m/e)^m) and the deviance
+// bd0(x, m) = x ln(x/m) + m - x computed without cancellation.  Only the
+// window |k - np| <= 12 sigma + 48 is kept: by Bernstein's inequality each
+// tail outside it holds at most exp(-72) of the mass for every sigma.  The
+// window is then normalised to sum 1 as the reference normalises its array.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double stirlerr(double n)
+{
+    const double S0 = 1.0 / 12.0, S1 = 1.0 / 360.0, S2 = 1.0 / 1260.0,
+                 S3 = 1.0 / 1680.0, S4 = 1.0 / 1188.0;
+    if (n <= 15.0)
+        return kStirlerr[(int)n];
+    const double nn = n * n;
+    return (S0 - (S1 - (S2 - (S3 - S4 / nn) / nn) / nn) / nn) / n;
+}
+
+__device__ __forceinline__ double bd0(double x, double np)
+{
+    if (fabs(x - np) < 0.1 * (x + np)) {
+        double v = (x - np) / (x + np);
+        double s = (x - np) * v;
+        double ej = 2.0 * x * v;
+        const double v2 = v * v;
+        for (int j = 1; j < 1000; ++j) {
+            ej *= v2;
+            const double s1 = s + ej / (double)(2 * j + 1);
+            if (s1 == s)
+                return s1;
+            s = s1;
+        }
+        return s;
+    }
+    return x * log(x / np) + np - x;
+}
+
+__device__ __forceinline__ double binom_pmf(double x, double n, double p)
+{
+    const double q = 1.0 - p;
+    if (p == 0.0)
+        return x == 0.0 ? 1.0 : 0.0;
+    if (q == 0.0)
+        return x == n ? 1.0 : 0.0;
+    if (x == 0.0) {
+        if (n == 0.0)
+            return 1.0;
+        const double lc = p < 0.1 ? -bd0(n, n * q) - n * p : n * log(q);
+        return exp(lc);
+    }
+    if (x == n) {
+        const double lc = q < 0.1 ? -bd0(n, n * p) - n * q : n * log(p);
+        return exp(lc);
+    }
+    const double lc = stirlerr(n) - stirlerr(x) - stirlerr(n - x) -
+                      bd0(x, n * p) - bd0(n - x, n * q);
+    const double lf = 1.8378770664093454836 /* ln 2 pi */ + log(x) +
+                      log1p(-x / n);
+    return exp(lc - 0.5 * lf);
+}
+
+__global__ __launch_bounds__(256) void binom_weights_kernel(
+    uint32_t n, const double *__restrict__ ps, const int64_t *__restrict__ start,
+    const int64_t *__restrict__ len, int64_t kmax, double *__restrict__ w)
+{
+    const uint32_t pi = blockIdx.y;
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= kmax)
+        return;
+    double v = 0.0;
+    if (k < len[pi])
+        v = binom_pmf((double)(start[pi] + k), (double)n, ps[pi]);
+    w[(int64_t)pi * kmax + k] = v;
+}
+
+// one block per p: divide the window by its sum
+__global__ __launch_bounds__(256) void binom_normalize_kernel(
+    int64_t kmax, const int64_t *__restrict__ len, double *__restrict__ w)
+{
+    __shared__ double part[256];
+    double *row = w + (int64_t)blockIdx.x * kmax;
+    const int64_t L = len[blockIdx.x];
+    double acc = 0.0;
+    for (int64_t k = threadIdx.x; k < L; k += 256)
+        acc += row[k];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double total = part[0];
+    if (total > 0.0)
+        for (int64_t k = threadIdx.x; k < L; k += 256)
+            row[k] /= total;
+}
+
+// ---------------------------------------------------------------------------
+// Canonical contraction Q[p][c] = sum_k w[p][k] X[start_p + k][c] in FP64 on
+// the vector ALUs: one block per p, threads stride over the window.
+// ---------------------------------------------------------------------------
+#define CONTRACT_MAX_COLS 32
+
+template <int C>
+__global__ __launch_bounds__(256) void contract_f64_kernel(
+    const double *__restrict__ x, int64_t x_stride, const int64_t *__restrict__ start,
+    const int64_t *__restrict__ len, const double *__restrict__ w, int64_t kmax,
+    double *__restrict__ out, int32_t cols)
+{
+    __shared__ double part[256 / 64][C];
+    const uint32_t pi = blockIdx.x;
+    const double *wr = w + (int64_t)pi * kmax;
+    const double *xr = x + start[pi] * x_stride;
+    const int64_t L = len[pi];
+    double acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        acc[c] = 0.0;
+    for (int64_t k = threadIdx.x; k < L; k += 256) {
+        const double wk = wr[k];
+        const double *row = xr + k * x_stride;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (c < cols)
+                acc[c] = fma(wk, row[c], acc[c]);
+    }
+    // wave reduction, then across the four waves
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        double v = acc[c];
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off, 64);
+        acc[c] = v;
+    }
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            part[wave][c] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)cols) {
+        const int c = threadIdx.x;
+        out[(int64_t)pi * cols + c] =
+            part[0][c] + part[1][c] + part[2][c] + part[3][c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Microcanonical mean and sample standard deviation from the exact sums:
+// mean = S1 / R, std = sqrt((R S2 - S1^2) / (R (R - 1))), with the numerator
+// formed exactly in 256-bit integers before the single rounding to FP64.
+// ---------------------------------------------------------------------------
+struct u256 {
+    uint64_t w[4];
+};
+
+__device__ __forceinline__ u256 mul_u192_u64(const uint64_t a[3], uint64_t b)
+{
+    u256 r;
+    u128 carry = 0;
+    for (int i = 0; i < 3; ++i) {
+        const u128 t = (u128)a[i] * b + carry;
+        r.w[i] = (uint64_t)t;
+        carry = t >> 64;
+    }
+    r.w[3] = (uint64_t)carry;
+    return r;
+}
+
+__device__ __forceinline__ u256 square_u128(uint64_t lo, uint64_t hi)
+{
+    u256 r;
+    const u128 ll = (u128)lo * lo, lh = (u128)lo * hi, hh = (u128)hi * hi;
+    r.w[0] = (uint64_t)ll;
+    u128 t = (ll >> 64) + (u128)(uint64_t)lh + (u128)(uint64_t)lh;
+    r.w[1] = (uint64_t)t;
+    t = (t >> 64) + (lh >> 64) + (lh >> 64) + (u128)(uint64_t)hh;
+    r.w[2] = (uint64_t)t;
+    t = (t >> 64) + (hh >> 64);
+    r.w[3] = (uint64_t)t;
+    return r;
+}
+
+__device__ __forceinline__ u256 sub_u256(const u256 &a, const u256 &b)
+{
+    u256 r;
+    uint64_t borrow = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint64_t d = a.w[i] - b.w[i];
+        const uint64_t d2 = d - borrow;
+        borrow = (a.w[i] < b.w[i]) || (d < borrow) ? 1u : 0u;
+        r.w[i] = d2;
+    }
+    return r;
+}
+
+__device__ __forceinline__ double u256_to_double(const u256 &a)
+{
+    // top 64 significant bits, rounded once to double, then scaled
+    int top = 3;
+    while (top > 0 && a.w[top] == 0)
+        --top;
+    if (top == 0)
+        return (double)a.w[0];
+    const int lz = __clzll(a.w[top]);
+    uint64_t m = lz ? (a.w[top] << lz) | (a.w[top - 1] >> (64 - lz)) : a.w[top];
+    // sticky bit for the discarded words
+    uint64_t rest = lz ? (a.w[top - 1] << lz) : a.w[top - 1];
+    for (int i = top - 2; i >= 0; --i)
+        rest |= a.w[i];
+    if (rest)
+        m |= 1u;
+    return ldexp((double)m, 64 * top - lz);
+}
+
+__device__ __forceinline__ double u128_div_to_double(uint64_t lo, uint64_t hi,
+                                                     uint64_t R)
+{
+    const u128 v = ((u128)hi << 64) | lo;
+    const u128 q = v / R, rem = v % R;
+    return (double)q + (double)(uint64_t)rem / (double)R;
+}
+
+__global__ __launch_bounds__(256) void micro_moments_kernel(
+    uint64_t total, uint64_t R, const uint64_t *__restrict__ sum1,
+    const uint64_t *__restrict__ sum2, double *__restrict__ mean,
+    double *__restrict__ std_out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total)
+        return;
+    const uint64_t lo = sum1[2 * i], hi = sum1[2 * i + 1];
+    mean[i] = u128_div_to_double(lo, hi, R);
+    if (R < 2) {
+        std_out[i] = __longlong_as_double(0x7FF8000000000000ll); // NaN, as ddof=1
+        return;
+    }
+    const uint64_t s2[3] = {sum2[3 * i], sum2[3 * i + 1], sum2[3 * i + 2]};
+    const u256 d = sub_u256(mul_u192_u64(s2, R), square_u128(lo, hi));
+    const double num = u256_to_double(d);
+    std_out[i] = sqrt(num / ((double)R * (double)(R - 1)));
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1105,5 +1356,99 @@ extern "C" int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges,
             num_nodes, num_edges, (const uint64_t *)d_records, rec_stride,
             d_span_step, d_chunk_state, d_max_size, d_spanning, d_moments);
     PC_LAUNCH_CHECK("expand_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_binomial_window(uint32_t n, double p, int64_t *start,
+                                  int64_t *len)
+{
+    g_last_error[0] = 0;
+    if (!start || !len)
+        return pc_fail(PC_EINVAL, "pc_binomial_window: NULL output");
+    if (!(p >= 0.0 && p <= 1.0))
+        return pc_fail(PC_EINVAL, "pc_binomial_window: p outside [0, 1]");
+    const double mu = (double)n * p;
+    const double sigma = sqrt((double)n * p * (1.0 - p));
+    const double half = 12.0 * sigma + 48.0;
+    double lo = floor(mu - half), hi = ceil(mu + half);
+    if (lo < 0.0)
+        lo = 0.0;
+    if (hi > (double)n)
+        hi = (double)n;
+    *start = (int64_t)lo;
+    *len = (int64_t)hi - (int64_t)lo + 1;
+    return PC_OK;
+}
+
+extern "C" int pc_binomial_weights(uint32_t n, uint32_t num_p,
+                                   const double *d_ps, const int64_t *d_start,
+                                   const int64_t *d_len, int64_t kmax,
+                                   double *d_w, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0)
+        return PC_OK;
+    if (!d_ps || !d_start || !d_len || !d_w || kmax <= 0)
+        return pc_fail(PC_EINVAL, "pc_binomial_weights: bad buffer");
+    hipStream_t s = as_stream(stream);
+    const dim3 grid((uint32_t)((kmax + 255) / 256), num_p);
+    binom_weights_kernel<<<grid, 256, 0, s>>>(n, d_ps, d_start, d_len, kmax, d_w);
+    PC_LAUNCH_CHECK("binom_weights_kernel");
+    binom_normalize_kernel<<<num_p, 256, 0, s>>>(kmax, d_len, d_w);
+    PC_LAUNCH_CHECK("binom_normalize_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
+                                     const double *d_x, int64_t x_stride,
+                                     uint32_t num_p, const int64_t *d_start,
+                                     const int64_t *d_len, const double *d_w,
+                                     int64_t kmax, double *d_out,
+                                     int32_t precision, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0 || cols == 0)
+        return PC_OK;
+    if (cols > CONTRACT_MAX_COLS)
+        return pc_fail(PC_EOVERFLOW, "pc_canonical_contract: more than %d columns",
+                       CONTRACT_MAX_COLS);
+    if (!d_x || !d_start || !d_len || !d_w || !d_out || x_stride < (int64_t)cols)
+        return pc_fail(PC_EINVAL, "pc_canonical_contract: bad buffer");
+    (void)num_steps;
+    hipStream_t s = as_stream(stream);
+    switch (precision) {
+    case PC_PREC_FP64:
+        if (cols <= 8)
+            contract_f64_kernel<8><<<num_p, 256, 0, s>>>(
+                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+        else if (cols <= 16)
+            contract_f64_kernel<16><<<num_p, 256, 0, s>>>(
+                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+        else
+            contract_f64_kernel<32><<<num_p, 256, 0, s>>>(
+                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+        PC_LAUNCH_CHECK("contract_f64_kernel");
+        return PC_OK;
+    default:
+        return pc_fail(PC_EINVAL, "pc_canonical_contract: unknown precision %d",
+                       precision);
+    }
+}
+
+extern "C" int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
+                                         const uint64_t *d_sum1,
+                                         const uint64_t *d_sum2, double *d_mean,
+                                         double *d_std, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0)
+        return pc_fail(PC_EINVAL, "pc_microcanonical_moments: no runs");
+    if (!d_sum1 || !d_sum2 || !d_mean || !d_std)
+        return pc_fail(PC_EINVAL, "pc_microcanonical_moments: NULL buffer");
+    const uint64_t total = (uint64_t)PC_COLUMNS * ((uint64_t)num_edges + 1u);
+    micro_moments_kernel<<<(uint32_t)((total + 255) / 256), 256, 0,
+                           as_stream(stream)>>>(total, num_runs, d_sum1, d_sum2,
+                                                d_mean, d_std);
+    PC_LAUNCH_CHECK("micro_moments_kernel");
     return PC_OK;
 }

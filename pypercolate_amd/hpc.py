@@ -1,0 +1,353 @@
+# encoding: utf-8
+"""Drop-in for ``percolate.hpc`` (andsor/pypercolate 0.4.6,
+percolate/hpc.py): the Newman-Ziff routines for HPC use, with the sweep and
+the convolution on the GPU.
+
+The structured-array layouts are the reference's (hpc.py:22-70, 407-440,
+518-558, 705-749) so results interoperate with code written against it.
+Integer outputs (max_cluster_size, has_spanning_cluster, moments modulo
+2**64) are bit-identical to the reference given the same edge order and
+numpy RandomState permutation.  Extensions for batches of runs carry a
+``_batch`` suffix.
+"""
+
+import numpy as np
+import scipy.stats
+
+from . import _graph
+
+
+def _ndarray_dtype(fields):
+    """NumPy structured dtype from (name, type) pairs (hpc.py:22-31)."""
+    return [(np.str_(key), values) for key, values in fields]
+
+
+def microcanonical_statistics_dtype(spanning_cluster=True):
+    """Dtype of sample states (hpc.py:34-70)."""
+    fields = [('n', 'uint32'), ('edge', 'uint32')]
+    if spanning_cluster:
+        fields.append(('has_spanning_cluster', 'bool'))
+    fields.extend([('max_cluster_size', 'uint32'), ('moments', '(5,)uint64')])
+    return _ndarray_dtype(fields)
+
+
+def canonical_statistics_dtype(spanning_cluster=True):
+    """Dtype of the canonical statistics of one run (hpc.py:407-440)."""
+    fields = []
+    if spanning_cluster:
+        fields.append(('percolation_probability', 'float64'))
+    fields.extend([('max_cluster_size', 'float64'), ('moments', '(5,)float64')])
+    return _ndarray_dtype(fields)
+
+
+def canonical_averages_dtype(spanning_cluster=True):
+    """Dtype of canonical averages over runs (hpc.py:518-558)."""
+    fields = [('number_of_runs', 'uint32')]
+    if spanning_cluster:
+        fields.extend([('percolation_probability_mean', 'float64'),
+                       ('percolation_probability_m2', 'float64')])
+    fields.extend([('max_cluster_size_mean', 'float64'),
+                   ('max_cluster_size_m2', 'float64'),
+                   ('moments_mean', '(5,)float64'),
+                   ('moments_m2', '(5,)float64')])
+    return _ndarray_dtype(fields)
+
+
+def finalized_canonical_averages_dtype(spanning_cluster=True):
+    """Dtype of finalized canonical averages (hpc.py:705-749)."""
+    fields = [('number_of_runs', 'uint32'), ('p', 'float64'), ('alpha', 'float64')]
+    if spanning_cluster:
+        fields.extend([('percolation_probability_mean', 'float64'),
+                       ('percolation_probability_std', 'float64'),
+                       ('percolation_probability_ci', '(2,)float64')])
+    fields.extend([('percolation_strength_mean', 'float64'),
+                   ('percolation_strength_std', 'float64'),
+                   ('percolation_strength_ci', '(2,)float64'),
+                   ('moments_mean', '(5,)float64'),
+                   ('moments_std', '(5,)float64'),
+                   ('moments_ci', '(5,2)float64')])
+    return _ndarray_dtype(fields)
+
+
+def _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
+                  auxiliary_edge_attributes, spanning_sides):
+    return _graph.from_percolation_graph(
+        perc_graph, spanning_cluster=spanning_cluster,
+        auxiliary_node_attributes=auxiliary_node_attributes,
+        auxiliary_edge_attributes=auxiliary_edge_attributes,
+        spanning_sides=spanning_sides)
+
+
+def _check_sides(spanning_cluster, spanning_sides):
+    if spanning_cluster and len(spanning_sides) != 2:
+        raise ValueError(
+            'Spanning cluster is to be detected, but auxiliary nodes '
+            'of less or more than 2 types (sides) given.'
+        )
+
+
+def _traces_to_structured(ga, perms, mx, sp, mom, spanning_cluster):
+    """(R, S) device-trace copies -> (R, S) structured arrays."""
+    R, S = mx.shape
+    out = np.empty((R, S), dtype=microcanonical_statistics_dtype(spanning_cluster))
+    out['n'] = np.arange(S, dtype=np.uint32)[np.newaxis, :]
+    out['edge'][:, 0] = 0
+    if S > 1:
+        out['edge'][:, 1:] = perms
+    if spanning_cluster:
+        out['has_spanning_cluster'] = sp
+    out['max_cluster_size'] = mx
+    out['moments'] = mom
+    return out
+
+
+def _sweep_host_perms(ga, perms, spanning_cluster):
+    """Run the GPU sweep for (R, M) permutations; structured (R, S) array."""
+    from ._engine import BondEngine
+    R = perms.shape[0]
+    S = ga.num_edges + 1
+    if ga.num_edges == 0:
+        mx = np.ones((R, 1), dtype=np.uint32)
+        sp = np.zeros((R, 1), dtype=bool)
+        mom = np.full((R, 1, 5), ga.num_nodes - 1, dtype=np.uint64)
+        return _traces_to_structured(ga, perms, mx, sp, mom, spanning_cluster)
+    eng = BondEngine(ga)
+    res = eng.sweep(eng.upload_permutations(perms))
+    mx, sp, mom = eng.host_traces(res)
+    assert mx.shape == (R, S)
+    return _traces_to_structured(ga, perms, mx, sp, mom, spanning_cluster)
+
+
+def bond_sample_states(
+    perc_graph, num_nodes, num_edges, seed, spanning_cluster=True,
+    auxiliary_node_attributes=None, auxiliary_edge_attributes=None,
+    spanning_sides=None,
+    **kwargs
+):
+    '''Generate successive sample states of the bond percolation model
+    (hpc.py:73-307).
+
+    Yields one structured row (dtype microcanonical_statistics_dtype) per
+    occupation number n = 0..M; as in the reference the same array object
+    is returned every time and overwritten by the next step.  The run's
+    permutation is numpy's RandomState(seed).permutation(num_edges); the run
+    itself is swept on the GPU at the second ``next()``.
+    '''
+    rng = np.random.RandomState(seed=seed)
+    if spanning_cluster:
+        _check_sides(spanning_cluster, spanning_sides)
+    perm_edges = rng.permutation(num_edges)
+
+    ret = np.empty(1, dtype=microcanonical_statistics_dtype(spanning_cluster))
+    ret['n'] = 0
+    ret['max_cluster_size'] = 1
+    ret['moments'] = np.ones(5, dtype='uint64') * (num_nodes - 1)
+    if spanning_cluster:
+        ret['has_spanning_cluster'] = False
+    yield ret
+
+    ga = _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
+                       auxiliary_edge_attributes, spanning_sides)
+    if ga.num_nodes != num_nodes or ga.num_edges != num_edges:
+        raise ValueError('num_nodes / num_edges do not match perc_graph')
+    trace = _sweep_host_perms(ga, perm_edges[np.newaxis, :], spanning_cluster)[0]
+    for n in range(1, num_edges + 1):
+        ret[0] = trace[n]
+        yield ret
+
+
+def bond_microcanonical_statistics(
+    perc_graph, num_nodes, num_edges, seed,
+    spanning_cluster=True,
+    auxiliary_node_attributes=None, auxiliary_edge_attributes=None,
+    spanning_sides=None,
+    **kwargs
+):
+    """Evolve a single run over all microstates (hpc.py:310-404): structured
+    array of num_edges + 1 rows.  ``ret['edge'][0]`` is 0 (undefined in the
+    reference)."""
+    return bond_microcanonical_statistics_batch(
+        perc_graph, num_nodes, num_edges, [seed], spanning_cluster,
+        auxiliary_node_attributes, auxiliary_edge_attributes, spanning_sides,
+    )[0]
+
+
+def bond_microcanonical_statistics_batch(
+    perc_graph, num_nodes, num_edges, seeds,
+    spanning_cluster=True,
+    auxiliary_node_attributes=None, auxiliary_edge_attributes=None,
+    spanning_sides=None,
+    **kwargs
+):
+    """``bond_microcanonical_statistics`` for many seeds in one GPU sweep:
+    structured array (len(seeds), num_edges + 1)."""
+    _check_sides(spanning_cluster, spanning_sides)
+    perms = np.stack([np.random.RandomState(seed=s).permutation(num_edges)
+                      for s in seeds]) if len(seeds) else \
+        np.zeros((0, num_edges), dtype=np.int64)
+    ga = _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
+                       auxiliary_edge_attributes, spanning_sides)
+    if ga.num_nodes != num_nodes or ga.num_edges != num_edges:
+        raise ValueError('num_nodes / num_edges do not match perc_graph')
+    return _sweep_host_perms(ga, perms.reshape(len(seeds), num_edges),
+                             spanning_cluster)
+
+
+def _trace_columns(microcanonical_statistics):
+    """(S, C) float64 columns of a run: [spanning,] max, moments 0..4."""
+    ms = microcanonical_statistics
+    cols = []
+    if 'has_spanning_cluster' in ms.dtype.names:
+        cols.append(ms['has_spanning_cluster'].astype(np.float64)[:, None])
+    cols.append(ms['max_cluster_size'].astype(np.float64)[:, None])
+    cols.append(ms['moments'].astype(np.float64))
+    return np.ascontiguousarray(np.concatenate(cols, axis=1))
+
+
+def bond_canonical_statistics(
+    microcanonical_statistics,
+    convolution_factors,
+    **kwargs
+):
+    """Canonical cluster statistics for a single run and a single p
+    (hpc.py:443-515): the convolution of the run's observables with the
+    factors, in FP64 on the GPU."""
+    import torch
+    from . import _engine as E
+    spanning_cluster = (
+        'has_spanning_cluster' in microcanonical_statistics.dtype.names
+    )
+    ret = np.empty(1, dtype=canonical_statistics_dtype(spanning_cluster))
+    x = _trace_columns(microcanonical_statistics)
+    f = np.asarray(convolution_factors, dtype=np.float64).reshape(-1)
+    if f.size != x.shape[0]:
+        raise ValueError('convolution_factors must have one entry per step')
+    wt = E.dense_weights(f)
+    q = E.contract(torch.from_numpy(x).to(wt.w.device), wt).cpu().numpy()[0]
+    off = 0
+    if spanning_cluster:
+        ret['percolation_probability'] = q[0]
+        off = 1
+    ret['max_cluster_size'] = q[off]
+    ret['moments'] = q[off + 1:off + 6]
+    return ret
+
+
+def bond_initialize_canonical_averages(
+    canonical_statistics, **kwargs
+):
+    """Canonical averages of a single run (hpc.py:561-635): number_of_runs 1,
+    means equal to the statistics, m2 zero."""
+    spanning_cluster = (
+        'percolation_probability' in canonical_statistics.dtype.names
+    )
+    ret = np.empty_like(
+        canonical_statistics,
+        dtype=canonical_averages_dtype(spanning_cluster=spanning_cluster),
+    )
+    ret['number_of_runs'] = 1
+    if spanning_cluster:
+        ret['percolation_probability_mean'] = (
+            canonical_statistics['percolation_probability'])
+        ret['percolation_probability_m2'] = 0.0
+    ret['max_cluster_size_mean'] = canonical_statistics['max_cluster_size']
+    ret['max_cluster_size_m2'] = 0.0
+    ret['moments_mean'] = canonical_statistics['moments']
+    ret['moments_m2'] = 0.0
+    return ret
+
+
+def _online_variance(a, b):
+    """Pairwise merge of (n, mean, M2) (Chan, Golub & LeVeque), the
+    algorithm behind the reference's simoa.stats.online_variance."""
+    n_a, mean_a, m2_a = a
+    n_b, mean_b, m2_b = b
+    n_a = np.asarray(n_a, dtype=np.float64)
+    n_b = np.asarray(n_b, dtype=np.float64)
+    n = n_a + n_b
+    delta = mean_b - mean_a
+    mean = mean_a + delta * (n_b / n)
+    m2 = m2_a + m2_b + delta * delta * (n_a * n_b / n)
+    return n, mean, m2
+
+
+def bond_reduce(row_a, row_b):
+    """Reduce the canonical averages over several runs (hpc.py:638-702):
+    associative and commutative merge of (n, mean, M2) per field."""
+    spanning_cluster = (
+        'percolation_probability_mean' in row_a.dtype.names and
+        'percolation_probability_mean' in row_b.dtype.names and
+        'percolation_probability_m2' in row_a.dtype.names and
+        'percolation_probability_m2' in row_b.dtype.names
+    )
+    ret = np.empty_like(row_a)
+
+    def _reducer(key, transpose=False):
+        mean_key = '{}_mean'.format(key)
+        m2_key = '{}_m2'.format(key)
+        na = row_a['number_of_runs']
+        nb = row_b['number_of_runs']
+        if transpose:
+            na = na[..., np.newaxis]
+            nb = nb[..., np.newaxis]
+        res = _online_variance((na, row_a[mean_key], row_a[m2_key]),
+                               (nb, row_b[mean_key], row_b[m2_key]))
+        ret[mean_key], ret[m2_key] = res[1], res[2]
+
+    if spanning_cluster:
+        _reducer('percolation_probability')
+    _reducer('max_cluster_size')
+    _reducer('moments', transpose=True)
+    ret['number_of_runs'] = row_a['number_of_runs'] + row_b['number_of_runs']
+    return ret
+
+
+def finalize_canonical_averages(
+    number_of_nodes, ps, canonical_averages, alpha,
+):
+    """Finalize canonical averages (hpc.py:752-834): mean, sample standard
+    deviation and t confidence interval; max_cluster_size becomes the
+    percolation strength and the moments are divided by the number of
+    nodes."""
+    spanning_cluster = (
+        ('percolation_probability_mean' in canonical_averages.dtype.names) and
+        'percolation_probability_m2' in canonical_averages.dtype.names
+    )
+    ret = np.empty_like(
+        canonical_averages,
+        dtype=finalized_canonical_averages_dtype(
+            spanning_cluster=spanning_cluster),
+    )
+    n = canonical_averages['number_of_runs']
+    sqrt_n = np.sqrt(canonical_averages['number_of_runs'])
+    ret['number_of_runs'] = n
+    ret['p'] = ps
+    ret['alpha'] = alpha
+
+    def _transform(original_key, final_key=None, normalize=False,
+                   transpose=False):
+        if final_key is None:
+            final_key = original_key
+        mean = canonical_averages['{}_mean'.format(original_key)]
+        m2 = canonical_averages['{}_m2'.format(original_key)]
+        nn = n[..., np.newaxis] if transpose else n
+        sq = sqrt_n[..., np.newaxis] if transpose else sqrt_n
+        mean_out = np.array(mean, dtype=np.float64)
+        if normalize:
+            mean_out = mean_out / number_of_nodes
+        std = np.sqrt(m2 / (nn - 1))
+        if normalize:
+            std = std / number_of_nodes
+        ret['{}_mean'.format(final_key)] = mean_out
+        ret['{}_std'.format(final_key)] = std
+        lo, hi = scipy.stats.t.interval(1 - alpha, df=nn - 1, loc=mean_out,
+                                        scale=std / sq)
+        ci = ret['{}_ci'.format(final_key)]
+        ci[..., 0] = lo
+        ci[..., 1] = hi
+
+    if spanning_cluster:
+        _transform('percolation_probability')
+    _transform('max_cluster_size', 'percolation_strength', normalize=True)
+    _transform('moments', normalize=True, transpose=True)
+    return ret

@@ -106,3 +106,426 @@ def _check_model_and_sides(graph, spanning_cluster, model):
                 'Spanning cluster is to be detected, but auxiliary nodes '
                 'of less or more than 2 types (sides) given.'
             )
+
+
+def _engine(graph_arrays):
+    from ._engine import BondEngine
+    return BondEngine(graph_arrays)
+
+
+def _run_batch(ga, perms):
+    """Sweep the given (R, M) host permutations on the GPU; returns the
+    engine and the SweepResult."""
+    eng = _engine(ga)
+    res = eng.sweep(eng.upload_permutations(perms))
+    return eng, res
+
+
+def sample_states(
+    graph, spanning_cluster=True, model='bond', copy_result=True
+):
+    '''Generate successive sample states of the percolation model
+    (percolate/percolate.py:103-356).
+
+    Yields a dict per occupation number n = 0..M with keys n, N, M,
+    max_cluster_size, moments (float64, 5), has_spanning_cluster (if
+    detected) and, from n = 1 on, edge (the node pair added).  As in the
+    reference the permutation is drawn from numpy's global random stream at
+    the second ``next()``; the whole run is then swept on the GPU.
+    '''
+    _check_model_and_sides(graph, spanning_cluster, model)
+    ga = _graph_arrays(graph, spanning_cluster)
+    N, M = ga.num_nodes, ga.num_edges
+
+    ret = dict()
+    ret['n'] = 0
+    ret['N'] = N
+    ret['M'] = M
+    ret['max_cluster_size'] = 1
+    ret['moments'] = np.ones(5) * (N - 1)
+    if spanning_cluster:
+        ret['has_spanning_cluster'] = False
+
+    if copy_result:
+        yield copy.deepcopy(ret)
+    else:
+        yield ret
+
+    perm_edges = np.random.permutation(M)
+    if M == 0:
+        return
+    eng, res = _run_batch(ga, perm_edges[np.newaxis, :])
+    mx, sp, mom = eng.host_traces(res)
+    moments = mom[0].astype(np.float64)
+    for n in range(1, M + 1):
+        ret['n'] = n
+        ret['edge'] = ga.edge_labels(perm_edges[n - 1])
+        ret['max_cluster_size'] = int(mx[0, n])
+        if copy_result:
+            ret['moments'] = moments[n].copy()
+        else:
+            ret['moments'][:] = moments[n]
+        if spanning_cluster:
+            ret['has_spanning_cluster'] = bool(sp[0, n])
+        if copy_result:
+            yield copy.deepcopy(ret)
+        else:
+            yield ret
+
+
+def single_run_arrays(spanning_cluster=True, **kwargs):
+    r'''Statistics of a single run as arrays (percolate/percolate.py:359-447):
+    N, M, max_cluster_size (M+1,), moments (5, M+1) and, with spanning
+    detection, has_spanning_cluster (M+1,).  Draws the permutation from
+    numpy's global random stream like ``sample_states``.'''
+    graph = kwargs.pop('graph')
+    model = kwargs.pop('model', 'bond')
+    kwargs.pop('copy_result', None)
+    if kwargs:
+        raise TypeError('unexpected arguments {}'.format(sorted(kwargs)))
+    _check_model_and_sides(graph, spanning_cluster, model)
+    ga = _graph_arrays(graph, spanning_cluster)
+    N, M = ga.num_nodes, ga.num_edges
+    perm = np.random.permutation(M)
+    ret = {'N': N, 'M': M}
+    if M:
+        eng, res = _run_batch(ga, perm[np.newaxis, :])
+        mx, sp, mom = eng.host_traces(res)
+    else:
+        mx = np.ones((1, 1), dtype=np.uint32)
+        sp = np.zeros((1, 1), dtype=bool)
+        mom = np.full((1, 1, 5), N - 1, dtype=np.uint64)
+    ret['max_cluster_size'] = mx[0].astype(np.float64)
+    ret['moments'] = mom[0].T.astype(np.float64)
+    if spanning_cluster:
+        ret['has_spanning_cluster'] = sp[0].astype(bool)
+    return ret
+
+
+def _t_interval(mean, std, runs, alpha):
+    """percolate.py:621-633 / 690-703 over arrays: the t interval where
+    std > 0, the mean twice where std == 0."""
+    mean = np.asarray(mean, dtype=np.float64)
+    std = np.asarray(std, dtype=np.float64)
+    ci = np.empty(mean.shape + (2,))
+    ci[..., 0] = mean
+    ci[..., 1] = mean
+    nz = std != 0
+    if runs > 1 and np.any(nz):
+        lo, hi = scipy.stats.t.interval(
+            1 - alpha, df=runs - 1, loc=mean[nz], scale=std[nz] / np.sqrt(runs))
+        ci[nz, 0] = lo
+        ci[nz, 1] = hi
+    return ci
+
+
+def _spanning_stats(k, runs, alpha):
+    """percolate.py:557-570: posterior mean (k+1)/(R+2) and the beta
+    credible interval, with one quantile call per distinct count k."""
+    k = np.asarray(k, dtype=np.float64)
+    mean = (k + 1) / (runs + 2)
+    ci = np.empty(k.shape + (2,))
+    uniq, inv = np.unique(k, return_inverse=True)
+    q = scipy.stats.beta.ppf(np.array([alpha / 2, 1 - alpha / 2])[:, None],
+                             uniq + 1, runs - uniq + 1)
+    ci[..., 0] = q[0][inv].reshape(k.shape)
+    ci[..., 1] = q[1][inv].reshape(k.shape)
+    return mean, ci
+
+
+class _MicrocanonicalStats(object):
+    """Per-step microcanonical statistics over R runs, computed on the GPU
+    from the exact sums (columns: max, spanning, moments 0..4)."""
+
+    def __init__(self, ga, runs, alpha, spanning_cluster, perms):
+        from . import _engine as E
+        self.N, self.M = ga.num_nodes, ga.num_edges
+        self.runs, self.alpha = runs, alpha
+        self.spanning = spanning_cluster
+        S = self.M + 1
+        if self.M:
+            eng, res = _run_batch(ga, perms)
+            s1, s2 = eng.microcanonical_sums(res)
+            mean, std = E.micro_moments(s1, s2, self.M, runs)
+            mean = mean.cpu().numpy()
+            std = std.cpu().numpy()
+            k_span = (s1[1, :, 0].cpu().numpy().view(np.uint64)).astype(np.float64)
+        else:
+            init = np.array([1.0, 0.0] + [N - 1.0 for N in [self.N] * 5])
+            mean = init[:, None] * np.ones((1, S))
+            std = np.zeros((7, S))
+            k_span = np.zeros(S)
+        self.max_mean = mean[0]
+        self.max_ci = _t_interval(mean[0], std[0], runs, alpha)
+        self.mom_mean = mean[2:7]
+        self.mom_ci = _t_interval(mean[2:7], std[2:7], runs, alpha)
+        if spanning_cluster:
+            self.span_mean, self.span_ci = _spanning_stats(k_span, runs, alpha)
+
+    def step(self, n, ret):
+        ret['n'] = n
+        ret['N'] = self.N
+        ret['M'] = self.M
+        ret['max_cluster_size'] = self.max_mean[n]
+        ret['max_cluster_size_ci'] = self.max_ci[n].copy()
+        ret['moments'] = self.mom_mean[:, n].copy()
+        ret['moments_ci'] = self.mom_ci[:, n].copy()
+        if self.spanning:
+            ret['spanning_cluster'] = self.span_mean[n]
+            ret['spanning_cluster_ci'] = self.span_ci[n].copy()
+        return ret
+
+    def arrays(self):
+        """microcanonical_averages_arrays of all steps (percolate.py:968-1064)."""
+        ret = {
+            'max_cluster_size': self.max_mean / self.N,
+            'max_cluster_size_ci': self.max_ci / self.N,
+            'moments': self.mom_mean / self.N,
+            'moments_ci': self.mom_ci / self.N,
+        }
+        if self.spanning:
+            ret['spanning_cluster'] = self.span_mean.copy()
+            ret['spanning_cluster_ci'] = self.span_ci.copy()
+        ret['M'] = self.M
+        ret['N'] = self.N
+        return ret
+
+
+def _initial_step_stats(N, M, runs, alpha, spanning_cluster):
+    """n = 0: every run has max 1, moments N - 1 and no spanning cluster."""
+    ret = {'n': 0, 'N': N, 'M': M, 'max_cluster_size': 1.0,
+           'max_cluster_size_ci': np.ones(2),
+           'moments': np.ones(5) * (N - 1),
+           'moments_ci': np.ones((5, 2)) * (N - 1)}
+    if spanning_cluster:
+        mean, ci = _spanning_stats(np.zeros(1), runs, alpha)
+        ret['spanning_cluster'] = mean[0]
+        ret['spanning_cluster_ci'] = ci[0]
+    return ret
+
+
+class _MicrocanonicalAverages(object):
+    """Iterator of microcanonical_averages (percolate.py:708-896).
+
+    Validation happens at the first ``next()``.  The n = 0 ensemble is
+    yielded before any permutation is drawn; at the second ``next()`` the
+    permutations of runs 0..R-1 are drawn in order from numpy's global
+    random stream (as the reference's lock-step zip over R generators does)
+    and all runs are swept and reduced on the GPU at once.
+    """
+
+    def __init__(self, graph, runs, spanning_cluster, model, alpha,
+                 copy_result):
+        self._args = (graph, runs, spanning_cluster, model, alpha)
+        self._copy = copy_result
+        self._n = None
+        self._stats = None
+        self._ret = dict()
+        self._ga = None
+
+    def __iter__(self):
+        return self
+
+    def _start(self):
+        graph, runs, spanning_cluster, model, alpha = self._args
+        try:
+            runs = int(runs)
+        except Exception:
+            raise ValueError("runs needs to be a positive integer")
+        if runs <= 0:
+            raise ValueError("runs needs to be a positive integer")
+        try:
+            alpha = float(alpha)
+        except Exception:
+            raise ValueError("alpha needs to be a float in the interval (0, 1)")
+        if alpha <= 0.0 or alpha >= 1.0:
+            raise ValueError("alpha needs to be a float in the interval (0, 1)")
+        _check_model_and_sides(graph, spanning_cluster, model)
+        self._args = (graph, runs, spanning_cluster, model, alpha)
+        self._ga = _graph_arrays(graph, spanning_cluster)
+
+    def _compute(self):
+        graph, runs, spanning_cluster, model, alpha = self._args
+        ga = self._ga
+        perms = np.stack([np.random.permutation(ga.num_edges)
+                          for _ in range(runs)])
+        self._stats = _MicrocanonicalStats(ga, runs, alpha, spanning_cluster,
+                                           perms)
+
+    def __next__(self):
+        if self._n is None:
+            self._start()
+            graph, runs, spanning_cluster, model, alpha = self._args
+            ret = _initial_step_stats(self._ga.num_nodes, self._ga.num_edges,
+                                      runs, alpha, spanning_cluster)
+            self._n = 0
+        else:
+            if self._n >= self._ga.num_edges:
+                raise StopIteration
+            if self._stats is None:
+                self._compute()
+            self._n += 1
+            ret = self._stats.step(self._n, dict())
+        if self._copy:
+            return copy.deepcopy(ret)
+        self._ret.clear()
+        self._ret.update(ret)
+        return self._ret
+
+    next = __next__
+
+    def _all_arrays(self):
+        """Fast path for microcanonical_averages_arrays on a fresh iterator."""
+        if self._n is not None:
+            return None
+        self._start()
+        self._n = 0
+        if self._ga.num_edges == 0:
+            return None
+        self._compute()
+        self._n = self._ga.num_edges
+        return self._stats.arrays()
+
+
+def microcanonical_averages(
+    graph, runs=40, spanning_cluster=True, model='bond', alpha=alpha_1sigma,
+    copy_result=True
+):
+    r'''Generate successive microcanonical percolation ensemble averages
+    (percolate/percolate.py:708-896): for n = 0..M the mean and confidence
+    interval over ``runs`` runs of max_cluster_size, moments (and the
+    Bayesian spanning_cluster estimate).  Raises ValueError for non-positive
+    runs or alpha outside (0, 1) at the first ``next()``.'''
+    return _MicrocanonicalAverages(graph, runs, spanning_cluster, model,
+                                   alpha, copy_result)
+
+
+def microcanonical_averages_arrays(microcanonical_averages):
+    """Compile microcanonical averages over all steps into arrays, all but
+    the spanning statistics divided by N (percolate/percolate.py:968-1064)."""
+    fast = getattr(microcanonical_averages, '_all_arrays', None)
+    if fast is not None:
+        arrays = fast()
+        if arrays is not None:
+            return arrays
+    ret = dict()
+    for n, microcanonical_average in enumerate(microcanonical_averages):
+        assert n == microcanonical_average['n']
+        if n == 0:
+            num_edges = microcanonical_average['M']
+            num_sites = microcanonical_average['N']
+            spanning_cluster = ('spanning_cluster' in microcanonical_average)
+            ret['max_cluster_size'] = np.empty(num_edges + 1)
+            ret['max_cluster_size_ci'] = np.empty((num_edges + 1, 2))
+            if spanning_cluster:
+                ret['spanning_cluster'] = np.empty(num_edges + 1)
+                ret['spanning_cluster_ci'] = np.empty((num_edges + 1, 2))
+            ret['moments'] = np.empty((5, num_edges + 1))
+            ret['moments_ci'] = np.empty((5, num_edges + 1, 2))
+        ret['max_cluster_size'][n] = microcanonical_average['max_cluster_size']
+        ret['max_cluster_size_ci'][n] = microcanonical_average['max_cluster_size_ci']
+        if spanning_cluster:
+            ret['spanning_cluster'][n] = microcanonical_average['spanning_cluster']
+            ret['spanning_cluster_ci'][n] = (
+                microcanonical_average['spanning_cluster_ci'])
+        ret['moments'][:, n] = microcanonical_average['moments']
+        ret['moments_ci'][:, n] = microcanonical_average['moments_ci']
+    for key in ret:
+        if 'spanning_cluster' in key:
+            continue
+        ret[key] /= num_sites
+    ret['M'] = num_edges
+    ret['N'] = num_sites
+    return ret
+
+
+def _binomial_pmf(n, p):
+    """Binomial PMF B(n, k; p) for k = 0..n (percolate/percolate.py:1067-1109),
+    computed on the GPU with Loader's saddle-point form over the window
+    |k - np| <= 12 sigma + 48 (zero outside; the truncated tails hold less
+    than 1e-31 of the mass) and normalised to sum 1."""
+    from . import _engine as E
+    n = int(n)
+    wt = E.binomial_weights(n, [p])
+    ret = np.zeros(n + 1)
+    start, length = int(wt.host_start[0]), int(wt.host_length[0])
+    ret[start:start + length] = wt.w[0, :length].cpu().numpy()
+    return ret
+
+
+_CANONICAL_KEYS = ('max_cluster_size', 'max_cluster_size_ci',
+                   'spanning_cluster', 'spanning_cluster_ci',
+                   'moments', 'moments_ci')
+
+
+def canonical_averages(ps, microcanonical_averages_arrays):
+    """Canonical cluster statistics from microcanonical ones
+    (percolate/percolate.py:1112-1223): every array (means and confidence
+    bounds) convolved with the binomial weights of each p, on the GPU in
+    FP64."""
+    from . import _engine as E
+    import torch
+    arrays = microcanonical_averages_arrays
+    num_sites = arrays['N']
+    num_edges = arrays['M']
+    spanning_cluster = ('spanning_cluster' in arrays)
+    ps = np.asarray(ps)
+
+    ret = dict()
+    ret['ps'] = ps
+    ret['N'] = num_sites
+    ret['M'] = num_edges
+
+    cols = []
+    layout = []
+    for key, value in arrays.items():
+        if len(key) <= 1:
+            continue
+        if key not in _CANONICAL_KEYS:
+            raise NotImplementedError('{}-dimensional array'.format(
+                np.ndim(value)))
+        v = np.asarray(value, dtype=np.float64)
+        if key in ('max_cluster_size', 'spanning_cluster'):
+            block = v.reshape(-1, 1)
+        elif key in ('max_cluster_size_ci', 'spanning_cluster_ci'):
+            block = v.reshape(-1, 2)
+        elif key == 'moments':
+            block = v.T
+        else:  # moments_ci (5, S, 2) -> (S, 10)
+            block = v.transpose(1, 0, 2).reshape(v.shape[1], -1)
+        layout.append((key, len(cols) and sum(c.shape[1] for c in cols), block.shape[1]))
+        cols.append(block)
+    x = np.ascontiguousarray(np.concatenate(cols, axis=1)) if cols else \
+        np.zeros((num_edges + 1, 0))
+    if ps.size and x.shape[1]:
+        wt = E.binomial_weights(num_edges, ps.reshape(-1))
+        dx = torch.from_numpy(x).to(wt.w.device)
+        q = E.contract(dx, wt).cpu().numpy()
+    else:
+        q = np.zeros((ps.size, x.shape[1]))
+    for key, off, width in layout:
+        block = q[:, off:off + width]
+        if key in ('max_cluster_size', 'spanning_cluster'):
+            ret[key] = block[:, 0].copy()
+        elif key in ('max_cluster_size_ci', 'spanning_cluster_ci'):
+            ret[key] = block.copy()
+        elif key == 'moments':
+            ret[key] = block.T.copy()
+        else:
+            ret[key] = block.reshape(ps.size, 5, 2).transpose(1, 0, 2).copy()
+    return ret
+
+
+def statistics(
+    graph, ps, spanning_cluster=True, model='bond', alpha=alpha_1sigma, runs=40
+):
+    """Helper function to compute percolation statistics
+    (percolate/percolate.py:1226-1252)."""
+    my_microcanonical_averages = microcanonical_averages(
+        graph=graph, runs=runs, spanning_cluster=spanning_cluster, model=model,
+        alpha=alpha
+    )
+    my_microcanonical_averages_arrays = microcanonical_averages_arrays(
+        my_microcanonical_averages
+    )
+    return canonical_averages(ps, my_microcanonical_averages_arrays)
