@@ -202,12 +202,19 @@ def binomial_windows(n, ps):
     return start, length
 
 
-def binomial_weights(n, ps):
-    """Device weights B(n, k; p), windowed and normalised, for every p."""
+def binomial_weights(n, ps, window=True):
+    """Device weights B(n, k; p), normalised, for every p: over the window
+    |k - np| <= 12 sigma + 48, or over all k = 0..n with window=False."""
     torch = nat.torch_cuda()
     lib = nat.load_library()
     ps = np.atleast_1d(np.asarray(ps, dtype=np.float64))
-    start, length = binomial_windows(n, ps)
+    if window:
+        start, length = binomial_windows(n, ps)
+    else:
+        if ps.size and not ((ps >= 0) & (ps <= 1)).all():
+            raise ValueError('p outside [0, 1]')
+        start = np.zeros(ps.size, dtype=np.int64)
+        length = np.full(ps.size, int(n) + 1, dtype=np.int64)
     kmax = int(length.max()) if ps.size else 1
     dev = torch.device('cuda', torch.cuda.current_device())
     d_ps = torch.from_numpy(ps.copy()).to(dev)

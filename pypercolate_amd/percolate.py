@@ -441,12 +441,12 @@ def microcanonical_averages_arrays(microcanonical_averages):
 
 def _binomial_pmf(n, p):
     """Binomial PMF B(n, k; p) for k = 0..n (percolate/percolate.py:1067-1109),
-    computed on the GPU with Loader's saddle-point form over the window
-    |k - np| <= 12 sigma + 48 (zero outside; the truncated tails hold less
-    than 1e-31 of the mass) and normalised to sum 1."""
+    computed on the GPU with Loader's saddle-point form for every k and
+    normalised to sum 1.  (canonical_averages uses the same weights over the
+    window |k - np| <= 12 sigma + 48 only.)"""
     from . import _engine as E
     n = int(n)
-    wt = E.binomial_weights(n, [p])
+    wt = E.binomial_weights(n, [p], window=False)
     ret = np.zeros(n + 1)
     start, length = int(wt.host_start[0]), int(wt.host_length[0])
     ret[start:start + length] = wt.w[0, :length].cpu().numpy()
