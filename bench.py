@@ -49,6 +49,19 @@ def parse_args(argv=None):
     return ap.parse_args(argv)
 
 
+def profiled_traffic(length, runs):
+    """HBM bytes per sweep launch from the committed PMC summary of this
+    workload (profiles/sweep_L<L>_R<R>.md), or None."""
+    path = os.path.join(ROOT, 'profiles', 'sweep_L{}_R{}.md'.format(length, runs))
+    try:
+        for line in open(path):
+            if line.startswith('sweep_traffic_bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(length, sample_runs, seed):
     """The oracle on a bounded sample: sweep + exact per-step sums."""
     import numpy as np
@@ -191,7 +204,7 @@ def main(argv=None):
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': achieved / HBM_PEAK_GBS,
-            'traffic': None,
+            'traffic': profiled_traffic(args.length, R),
             'alg_bytes_per_add': alg_bytes,
             'kernel_ms': sweep_avg_ms,
         },
