@@ -121,8 +121,13 @@ int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
                      uint32_t *d_max_size, uint8_t *d_spanning,
                      uint64_t *d_moments, void *stream);
 
-/* Precisions of pc_canonical_contract. */
-#define PC_PREC_FP64 0      /* FP64 FMA on the vector ALUs                 */
+/* Precisions of pc_canonical_contract (element type of d_x and d_w). */
+#define PC_PREC_FP64 0        /* FP64 inputs, FP64 FMA on the vector ALUs    */
+#define PC_PREC_FP64_MFMA 1   /* FP64 inputs, v_mfma_f64_16x16x4f64          */
+#define PC_PREC_FP32_MFMA 2   /* FP32 inputs, v_mfma_f32_16x16x4f32, FP32
+                                 accumulator added into FP64 every 8 MFMAs  */
+#define PC_PREC_BF16X3_MFMA 3 /* FP32 inputs split into three bf16 terms,
+                                 six products on v_mfma_f32_16x16x32_bf16  */
 
 /*
  * Convolution window of the binomial weights B(n, k; p): k in
@@ -142,17 +147,29 @@ int pc_binomial_weights(uint32_t n, uint32_t num_p, const double *d_ps,
                         const int64_t *d_start, const int64_t *d_len,
                         int64_t kmax, double *d_w, void *stream);
 
+/* Device workspace bytes pc_canonical_contract needs at a precision. */
+int pc_contract_workspace_size(uint32_t num_p, uint32_t cols, int32_t precision,
+                               uint64_t *bytes);
+
 /*
  * Canonical averages, the convolution of percolate/percolate.py:1196-1221
  * and percolate/hpc.py:496-512: d_out[i * cols + c] =
  * sum_j d_w[i * kmax + j] * d_x[(d_start[i] + j) * x_stride + c] for
- * c < cols <= 32, at the given precision (PC_PREC_*).
+ * c < cols <= 32 and rows < num_steps, at the given precision (PC_PREC_*;
+ * d_x and d_w are double for the FP64 forms, float otherwise).  The MFMA
+ * forms sum fixed slices of the steps into d_workspace and add them in a
+ * fixed order (deterministic).
  */
-int pc_canonical_contract(uint32_t num_steps, uint32_t cols, const double *d_x,
+int pc_canonical_contract(uint32_t num_steps, uint32_t cols, const void *d_x,
                           int64_t x_stride, uint32_t num_p,
                           const int64_t *d_start, const int64_t *d_len,
-                          const double *d_w, int64_t kmax, double *d_out,
-                          int32_t precision, void *stream);
+                          const void *d_w, int64_t kmax, double *d_out,
+                          int32_t precision, void *d_workspace,
+                          uint64_t workspace_bytes, void *stream);
+
+/* FP64 -> FP32 conversion of n elements (inputs of the FP32 precisions). */
+int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
+                          void *stream);
 
 /*
  * Mean and sample standard deviation (ddof = 1) over num_runs runs of each

@@ -244,25 +244,66 @@ def dense_weights(factors):
     return wt
 
 
+def to_f32(t):
+    """FP64 device tensor -> FP32 copy (pc_convert_f64_to_f32)."""
+    torch = nat.torch_cuda()
+    t = t.contiguous()
+    out = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+    nat.check(nat.load_library().pc_convert_f64_to_f32(
+        nat.ptr(t), t.numel(), nat.ptr(out), nat.stream_handle(torch)),
+        'pc_convert_f64_to_f32')
+    return out
+
+
+class ContractPlan(object):
+    """Inputs of one contraction at one precision, converted once (FP32
+    forms read FP32 copies of x and of the weights) with the workspace
+    allocated: call run() as often as needed."""
+
+    def __init__(self, x, weights, precision=nat.PC_PREC_FP64):
+        torch = nat.torch_cuda()
+        self.lib = nat.load_library()
+        self.precision = nat.PRECISIONS.get(precision, precision)
+        x = x.contiguous()
+        self.S, self.C = int(x.shape[0]), int(x.shape[1])
+        self.P = (int(weights.w.shape[0]) if weights.ps is None
+                  else len(weights.ps))
+        if self.P and self.S and (
+                int(weights.host_start.min()) < 0 or
+                int((weights.host_start + weights.host_length).max()) > self.S):
+            raise ValueError('convolution window outside the {} steps'.format(self.S))
+        fp32 = self.precision in (nat.PC_PREC_FP32_MFMA, nat.PC_PREC_BF16X3_MFMA)
+        self.x = to_f32(x) if fp32 else x
+        self.w = to_f32(weights.w) if fp32 else weights.w
+        self.weights = weights
+        b = ctypes.c_uint64(0)
+        nat.check(self.lib.pc_contract_workspace_size(self.P, self.C,
+                                                      self.precision,
+                                                      ctypes.byref(b)),
+                  'pc_contract_workspace_size')
+        self.ws_bytes = int(b.value)
+        self.ws = (torch.empty((self.ws_bytes,), dtype=torch.uint8,
+                               device=x.device) if self.ws_bytes else None)
+        self.out = torch.empty((self.P, self.C), dtype=torch.float64,
+                               device=x.device)
+
+    def run(self):
+        torch = nat.torch_cuda()
+        if self.P and self.S and self.C:
+            wt = self.weights
+            nat.check(self.lib.pc_canonical_contract(
+                self.S, self.C, nat.ptr(self.x), self.C, self.P,
+                nat.ptr(wt.start), nat.ptr(wt.length), nat.ptr(self.w),
+                wt.kmax, nat.ptr(self.out), self.precision, nat.ptr(self.ws),
+                self.ws_bytes, nat.stream_handle(torch)),
+                'pc_canonical_contract')
+        return self.out
+
+
 def contract(x, weights, precision=nat.PC_PREC_FP64):
     """Canonical convolution of the columns of x (S, C) float64 device
     tensor with every weight row: (P, C) float64 device tensor."""
-    torch = nat.torch_cuda()
-    lib = nat.load_library()
-    x = x.contiguous()
-    S, C = int(x.shape[0]), int(x.shape[1])
-    P = int(weights.w.shape[0]) if weights.ps is None else len(weights.ps)
-    out = torch.empty((P, C), dtype=torch.float64, device=x.device)
-    if P and S:
-        if int(weights.host_start.min()) < 0 or int(
-                (weights.host_start + weights.host_length).max()) > S:
-            raise ValueError('convolution window outside the {} steps'.format(S))
-        nat.check(lib.pc_canonical_contract(
-            S, C, nat.ptr(x), C, P, nat.ptr(weights.start),
-            nat.ptr(weights.length), nat.ptr(weights.w), weights.kmax,
-            nat.ptr(out), int(precision), nat.stream_handle(torch)),
-            'pc_canonical_contract')
-    return out
+    return ContractPlan(x, weights, precision).run()
 
 
 def micro_moments(sum1, sum2, num_edges, num_runs):

@@ -961,6 +961,7 @@ __global__ __launch_bounds__(256) void binom_normalize_kernel(
 // the vector ALUs: one block per p, threads stride over the window.
 // ---------------------------------------------------------------------------
 #define CONTRACT_MAX_COLS 32
+#define PC_CONTRACT_SLICES 16
 
 template <int C>
 __global__ __launch_bounds__(256) void contract_f64_kernel(
@@ -1004,6 +1005,186 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(
         out[(int64_t)pi * cols + c] =
             part[0][c] + part[1][c] + part[2][c] + part[3][c];
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// Canonical contraction on the matrix cores.  One wavefront computes a tile
+// of 16 p-values x 16 columns over one slice of the steps (the union of the
+// 16 windows is split across blockIdx.z for parallelism); slices write
+// partial sums that contract_reduce_kernel adds in a fixed order, so the
+// result does not depend on scheduling.
+//   FP64:   v_mfma_f64_16x16x4f64, FP64 in, FP64 accumulate.
+//   FP32:   v_mfma_f32_16x16x4f32 on FP32 inputs; the FP32 accumulator is
+//           added into FP64 every 8 MFMAs (bounded FP32 accumulation).
+//   BF16X3: FP32 inputs split into three bf16 terms hi + mid + lo, the six
+//           leading products (lo*hi, hi*lo, mid*mid, mid*hi, hi*mid,
+//           hi*hi) on v_mfma_f32_16x16x32_bf16, FP32 accumulator added into
+//           FP64 every 8 steps.
+// Lane layout of the 16x16xK MFMAs: A row i = lane % 16 holds K/4
+// consecutive k starting at (lane / 16) * K/4; B column j = lane % 16 the
+// same k; register r of the output holds D[4 * (lane / 16) + r][lane % 16]
+// for the FP32 accumulators and D[lane / 16 + 4 r][lane % 16] for FP64.
+// ---------------------------------------------------------------------------
+typedef double v4f64 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
+
+#define MFMA_FLUSH 8
+
+struct TileCtx {
+    uint32_t P, C;
+    int64_t x_stride, kmax;
+    const int64_t *start, *len;
+};
+
+__device__ __forceinline__ int64_t tile_lo(const TileCtx &t, uint32_t p0)
+{
+    int64_t lo = INT64_MAX;
+    for (uint32_t i = 0; i < 16 && p0 + i < t.P; ++i)
+        lo = min(lo, t.start[p0 + i]);
+    return lo;
+}
+__device__ __forceinline__ int64_t tile_hi(const TileCtx &t, uint32_t p0)
+{
+    int64_t hi = 0;
+    for (uint32_t i = 0; i < 16 && p0 + i < t.P; ++i)
+        hi = max(hi, t.start[p0 + i] + t.len[p0 + i]);
+    return hi;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_w(const T *__restrict__ w, const TileCtx &t,
+                                    uint32_t p, int64_t g)
+{
+    if (p >= t.P)
+        return (T)0;
+    const int64_t k = g - t.start[p];
+    return (k >= 0 && k < t.len[p]) ? w[(int64_t)p * t.kmax + k] : (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_x(const T *__restrict__ x, const TileCtx &t,
+                                    int64_t g, uint32_t c, int64_t S)
+{
+    return (c < t.C && g < S) ? x[g * t.x_stride + c] : (T)0;
+}
+
+__device__ __forceinline__ void split3(float v, __bf16 &hi, __bf16 &mid, __bf16 &lo)
+{
+    hi = (__bf16)v;
+    const float r1 = v - (float)hi;
+    mid = (__bf16)r1;
+    const float r2 = r1 - (float)mid;
+    lo = (__bf16)r2;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(64) void contract_mfma_kernel(
+    const void *__restrict__ xv, int64_t S, const void *__restrict__ wv,
+    TileCtx t, uint32_t slices, double *__restrict__ partial)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t p0 = blockIdx.x * 16u, c0 = blockIdx.y * 16u;
+    const uint32_t slice = blockIdx.z;
+    const int64_t lo = tile_lo(t, p0), hi = tile_hi(t, p0);
+    constexpr int64_t KSTEP = (PREC == PC_PREC_BF16X3_MFMA) ? 32 : 4;
+    const int64_t steps = hi > lo ? (hi - lo + KSTEP - 1) / KSTEP : 0;
+    const int64_t per = (steps + slices - 1) / slices;
+    const int64_t g_begin = lo + (int64_t)slice * per * KSTEP;
+    const int64_t g_end = min(hi, g_begin + per * KSTEP);
+    const uint32_t i = lane % 16u, q = lane / 16u;
+    const uint32_t p = p0 + i, c = c0 + i;
+
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    if (PREC == PC_PREC_FP64_MFMA) {
+        const double *w = (const double *)wv, *x = (const double *)xv;
+        v4f64 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int64_t g = g_begin; g < g_end; g += 4) {
+            const double a = load_w(w, t, p, g + q);
+            const double b = load_x(x, t, g + q, c, S);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        for (int r = 0; r < 4; ++r)
+            out[r] = acc[r];
+    } else if (PREC == PC_PREC_FP32_MFMA) {
+        const float *w = (const float *)wv, *x = (const float *)xv;
+        v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+        int n = 0;
+        for (int64_t g = g_begin; g < g_end; g += 4) {
+            const float a = load_w(w, t, p, g + q);
+            const float b = load_x(x, t, g + q, c, S);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            if (++n == MFMA_FLUSH) {
+                for (int r = 0; r < 4; ++r)
+                    out[r] += (double)acc[r];
+                acc = v4f32{0.f, 0.f, 0.f, 0.f};
+                n = 0;
+            }
+        }
+        for (int r = 0; r < 4; ++r)
+            out[r] += (double)acc[r];
+    } else {
+        const float *w = (const float *)wv, *x = (const float *)xv;
+        v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+        int n = 0;
+        for (int64_t g = g_begin; g < g_end; g += 32) {
+            v8bf16 ah, am, al, bh, bm, bl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t gk = g + 8 * q + e;
+                __bf16 h, m, l;
+                split3(load_w(w, t, p, gk), h, m, l);
+                ah[e] = h; am[e] = m; al[e] = l;
+                split3(load_x(x, t, gk, c, S), h, m, l);
+                bh[e] = h; bm[e] = m; bl[e] = l;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+            if (++n == MFMA_FLUSH) {
+                for (int r = 0; r < 4; ++r)
+                    out[r] += (double)acc[r];
+                acc = v4f32{0.f, 0.f, 0.f, 0.f};
+                n = 0;
+            }
+        }
+        for (int r = 0; r < 4; ++r)
+            out[r] += (double)acc[r];
+    }
+    // output rows: D[4q + r][i] for the FP32-accumulating MFMAs, rows
+    // interleaved as D[q + 4r][i] for v_mfma_f64_16x16x4f64
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t row = (PREC == PC_PREC_FP64_MFMA) ? q + 4u * (uint32_t)r
+                                                         : 4u * q + (uint32_t)r;
+        const uint32_t pr = p0 + row, cc = c0 + i;
+        if (pr < t.P && cc < t.C)
+            partial[((int64_t)slice * t.P + pr) * t.C + cc] = out[r];
+    }
+}
+
+__global__ __launch_bounds__(256) void contract_reduce_kernel(
+    const double *__restrict__ partial, uint32_t slices, int64_t n,
+    double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    double acc = 0.0;
+    for (uint32_t s = 0; s < slices; ++s)
+        acc += partial[(int64_t)s * n + i];
+    out[i] = acc;
+}
+
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(
+    const double *__restrict__ in, int64_t n, float *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        out[i] = (float)in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1399,12 +1580,25 @@ extern "C" int pc_binomial_weights(uint32_t n, uint32_t num_p,
     return PC_OK;
 }
 
+extern "C" int pc_contract_workspace_size(uint32_t num_p, uint32_t cols,
+                                          int32_t precision, uint64_t *bytes)
+{
+    g_last_error[0] = 0;
+    if (!bytes)
+        return pc_fail(PC_EINVAL, "pc_contract_workspace_size: bytes is NULL");
+    *bytes = precision == PC_PREC_FP64
+                 ? 0u
+                 : (uint64_t)PC_CONTRACT_SLICES * num_p * cols * sizeof(double);
+    return PC_OK;
+}
+
 extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
-                                     const double *d_x, int64_t x_stride,
+                                     const void *d_x, int64_t x_stride,
                                      uint32_t num_p, const int64_t *d_start,
-                                     const int64_t *d_len, const double *d_w,
+                                     const int64_t *d_len, const void *d_w,
                                      int64_t kmax, double *d_out,
-                                     int32_t precision, void *stream)
+                                     int32_t precision, void *d_workspace,
+                                     uint64_t workspace_bytes, void *stream)
 {
     g_last_error[0] = 0;
     if (num_p == 0 || cols == 0)
@@ -1414,25 +1608,65 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
                        CONTRACT_MAX_COLS);
     if (!d_x || !d_start || !d_len || !d_w || !d_out || x_stride < (int64_t)cols)
         return pc_fail(PC_EINVAL, "pc_canonical_contract: bad buffer");
-    (void)num_steps;
     hipStream_t s = as_stream(stream);
-    switch (precision) {
-    case PC_PREC_FP64:
+    if (precision == PC_PREC_FP64) {
+        const double *x = (const double *)d_x, *w = (const double *)d_w;
         if (cols <= 8)
             contract_f64_kernel<8><<<num_p, 256, 0, s>>>(
-                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+                x, x_stride, d_start, d_len, w, kmax, d_out, (int32_t)cols);
         else if (cols <= 16)
             contract_f64_kernel<16><<<num_p, 256, 0, s>>>(
-                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+                x, x_stride, d_start, d_len, w, kmax, d_out, (int32_t)cols);
         else
             contract_f64_kernel<32><<<num_p, 256, 0, s>>>(
-                d_x, x_stride, d_start, d_len, d_w, kmax, d_out, (int32_t)cols);
+                x, x_stride, d_start, d_len, w, kmax, d_out, (int32_t)cols);
         PC_LAUNCH_CHECK("contract_f64_kernel");
         return PC_OK;
+    }
+    uint64_t need = 0;
+    pc_contract_workspace_size(num_p, cols, precision, &need);
+    if (!d_workspace || workspace_bytes < need)
+        return pc_fail(PC_ENOMEM, "pc_canonical_contract: workspace needs %llu bytes",
+                       (unsigned long long)need);
+    const TileCtx t{num_p, cols, x_stride, kmax, d_start, d_len};
+    const dim3 grid((num_p + 15u) / 16u, (cols + 15u) / 16u, PC_CONTRACT_SLICES);
+    double *partial = (double *)d_workspace;
+    switch (precision) {
+    case PC_PREC_FP64_MFMA:
+        contract_mfma_kernel<PC_PREC_FP64_MFMA><<<grid, 64, 0, s>>>(
+            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+        break;
+    case PC_PREC_FP32_MFMA:
+        contract_mfma_kernel<PC_PREC_FP32_MFMA><<<grid, 64, 0, s>>>(
+            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+        break;
+    case PC_PREC_BF16X3_MFMA:
+        contract_mfma_kernel<PC_PREC_BF16X3_MFMA><<<grid, 64, 0, s>>>(
+            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+        break;
     default:
         return pc_fail(PC_EINVAL, "pc_canonical_contract: unknown precision %d",
                        precision);
     }
+    PC_LAUNCH_CHECK("contract_mfma_kernel");
+    const int64_t n = (int64_t)num_p * cols;
+    contract_reduce_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(
+        partial, PC_CONTRACT_SLICES, n, d_out);
+    PC_LAUNCH_CHECK("contract_reduce_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
+                                     void *stream)
+{
+    g_last_error[0] = 0;
+    if (n <= 0)
+        return PC_OK;
+    if (!d_in || !d_out)
+        return pc_fail(PC_EINVAL, "pc_convert_f64_to_f32: NULL buffer");
+    f64_to_f32_kernel<<<2048, 256, 0, as_stream(stream)>>>(d_in, n, d_out);
+    PC_LAUNCH_CHECK("f64_to_f32_kernel");
+    return PC_OK;
 }
 
 extern "C" int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
