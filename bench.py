@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -46,6 +48,13 @@ def parse_args(argv=None):
                     help='runs in the CPU-oracle sample (0 skips it)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a 1 GiB device copy (for PMC scaling)')
+    ap.add_argument('--canonical-length', type=int, default=1024,
+                    help='lattice of the canonical-average comparison (0 skips)')
+    ap.add_argument('--canonical-runs', type=int, default=64)
+    ap.add_argument('--canonical-ps', type=int, default=4096)
+    ap.add_argument('--batch', action='store_true',
+                    help='time the finite-size batch (L = 16..512, 2^26/L^2 '
+                         'runs each, one HIP graph) instead of L = 128')
     return ap.parse_args(argv)
 
 
@@ -60,6 +69,105 @@ def profiled_traffic(length, runs):
     except OSError:
         pass
     return None
+
+
+def canonical_comparison(length, runs, num_p, seed=7, reps=5):
+    """The canonical average at every precision on real microcanonical data:
+    L x L lattice, `runs` runs (device permutations), 14 columns (mean and
+    mean square of the 7 observables per step), num_p values of p in
+    [0.3, 0.7].  Returns per precision the median contraction time (HIP
+    events) and the largest error per column relative to sum |w x| against
+    the FP64 vector-ALU form."""
+    import numpy as np
+    import torch
+    from pypercolate_amd import _graph, _native as nat
+    from pypercolate_amd import _engine as E
+
+    eng = E.BondEngine(_graph.square_lattice(length))
+    res = eng.sweep(eng.device_permutations(seed, runs))
+    s1, s2 = eng.microcanonical_sums(res)
+    mean, std = E.micro_moments(s1, s2, eng.M, runs)
+    second = std * std * ((runs - 1.0) / runs) + mean * mean
+    x = torch.cat([mean, second], dim=0).t().contiguous()   # (S, 14)
+    ps = np.linspace(0.3, 0.7, num_p)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    wt = E.binomial_weights(eng.M, ps)
+    t1.record()
+    torch.cuda.synchronize()
+    weights_ms = t0.elapsed_time(t1)
+    lens = wt.host_length.astype(np.float64)
+    flops = 2.0 * lens.sum() * x.shape[1]
+    mag = E.ContractPlan(x.abs(), wt, 'fp64').run().cpu().numpy()
+    out = {'workload': ('L={} (M={}), {} runs, {} p in [0.3, 0.7], 14 columns, '
+                        'windows |k - Mp| <= 12 sigma + 48 (mean {:.0f} steps)'
+                        ).format(length, eng.M, runs, num_p, lens.mean()),
+           'weights_ms': weights_ms, 'algorithmic_flops': flops, 'forms': {}}
+    ref = None
+    for name in ('fp64', 'fp64_mfma', 'fp32_mfma', 'bf16x3_mfma'):
+        plan = E.ContractPlan(x, wt, name)
+        plan.run()
+        times = []
+        for _ in range(reps):
+            t0.record()
+            plan.run()
+            t1.record()
+            torch.cuda.synchronize()
+            times.append(t0.elapsed_time(t1))
+        q = plan.run().cpu().numpy()
+        if ref is None:
+            ref = q
+        err = (np.abs(q - ref) / np.where(mag > 0, mag, 1.0)).max(axis=0)
+        ms = sorted(times)[len(times) // 2]
+        out['forms'][name] = {'ms': ms, 'tflops': flops / ms / 1e9,
+                              'max_rel_err_per_column': [float(e) for e in err],
+                              'max_rel_err': float(err.max())}
+    base = out['forms']['fp64']['ms']
+    for name, f in out['forms'].items():
+        f['speedup_vs_fp64'] = base / f['ms']
+    return out
+
+
+def batch_bench(args):
+    """Finite-size batch: L in {16, 32, 64, 128, 256, 512}, 2^26 / L^2 runs
+    each (Philox seed 42), permute -> sweep -> reduce -> convolve (101 p in
+    [0.4, 0.6]) for every L, recorded once as a HIP graph and replayed."""
+    import torch
+    from pypercolate_amd.batch import FiniteSizeBatch, default_runs
+    lengths = [16, 32, 64, 128, 256, 512]
+    runs = default_runs(lengths)
+    batch = FiniteSizeBatch(lengths, runs, seed=42, ps=np.linspace(0.4, 0.6, 101))
+    batch.capture()
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        batch.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    res = batch.results()
+    ok = all(int(r['sum1'][0, -1, 0]) == r['runs'] * L * L for L, r in res.items())
+    return {
+        'metric': METRIC + ' [finite-size batch]',
+        'value': batch.additions / (ms / 1e3),
+        'unit': 'bond additions/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'u16/u32/u64/u128 + f64',
+        'data': 'synthetic: Philox-keyed device permutations, seed 42',
+        'config': {'workload': ('finite-size batch, L in {} with {} runs, '
+                                'one HIP graph per step').format(lengths, runs),
+                   'lengths': lengths, 'runs': runs,
+                   'parallelism': 'single GPU, one stream per L'},
+        'tuning': {L: r['tuning'] for L, r in res.items()},
+        'sums_check': ok,
+    }
 
 
 def cpu_baseline(length, sample_runs, seed):
@@ -113,6 +221,10 @@ def main(argv=None):
     from pypercolate_amd._engine import BondEngine
 
     torch.cuda.set_device(0)
+    if args.batch:
+        out = batch_bench(args)
+        print(json.dumps(out), flush=True)
+        return 0 if out['sums_check'] else 1
     eng = BondEngine(_graph.square_lattice(args.length))
     R, M, N, S = args.runs, eng.M, eng.N, eng.S
     dev = eng.device
@@ -211,6 +323,9 @@ def main(argv=None):
         'sums_check': ok,
         'wall_s': t_wall,
     }
+    if args.canonical_length > 0:
+        out['canonical'] = canonical_comparison(
+            args.canonical_length, args.canonical_runs, args.canonical_ps)
     if args.cpu_sample_runs > 0:
         out['cpu_baseline'] = cpu_baseline(args.length, args.cpu_sample_runs,
                                            args.seed)

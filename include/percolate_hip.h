@@ -66,23 +66,36 @@ int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
                       uint32_t num_edges, int32_t *d_perm, int64_t perm_stride,
                       void *stream);
 
-/* Device workspace bytes pc_sweep needs (0 when the state fits in LDS). */
+/* Where pc_sweep keeps the union-find state of a run. */
+#define PC_STATE_AUTO 0 /* LDS when num_nodes <= 16384, else HBM             */
+#define PC_STATE_LDS 1  /* 16-bit codes in LDS (num_nodes <= 16384)          */
+#define PC_STATE_HBM 2  /* 32-bit codes in the HBM workspace                 */
+
+/* Device workspace bytes pc_sweep needs (0 when the state is in LDS). */
 int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
-                            uint64_t *bytes);
+                            uint32_t state_mode, uint64_t *bytes);
 
 /*
  * The Newman-Ziff sweep of num_runs independent runs, one per wavefront:
  * replaces the per-edge loop of percolate/hpc.py:249-307 (and its twin
  * percolate/percolate.py:298-356).  Reads d_perm[r * perm_stride + n],
  * writes merge records d_records[r * rec_stride + n] (n < num_edges) and
- * d_span_step[r].  Union-find state lives in LDS (num_nodes <= 16384) or in
- * d_workspace (pc_sweep_workspace_size bytes).
+ * d_span_step[r].  Union-find state lives in LDS or in d_workspace
+ * (pc_sweep_workspace_size bytes) per state_mode (PC_STATE_*).
+ * merge_rounds is the number of parallel conflict-free merge rounds per
+ * batch of 64 bonds before the in-order remainder (0 = library default);
+ * results do not depend on it or on state_mode.
  */
 int pc_sweep(uint32_t num_nodes, uint32_t num_edges, const int32_t *d_edges,
              const uint8_t *d_node_sides, int32_t span_on_first_merge,
              const int32_t *d_perm, int64_t perm_stride, uint32_t num_runs,
              void *d_records, int64_t rec_stride, uint32_t *d_span_step,
-             void *d_workspace, uint64_t workspace_bytes, void *stream);
+             void *d_workspace, uint64_t workspace_bytes,
+             uint32_t merge_rounds, uint32_t state_mode, void *stream);
+
+/* Resident sweep workgroups (runs) per compute unit for a graph size. */
+int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
+                       int32_t *blocks_per_cu);
 
 /*
  * Per-run state at the start of every chunk of PC_CHUNK steps, from the

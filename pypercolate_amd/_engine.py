@@ -58,6 +58,9 @@ class BondEngine(object):
         self.rec_bytes = self.lib.pc_record_bytes(self.N)
         self.rec_torch = torch.int32 if self.rec_bytes == 4 else torch.int64
         self.rec_numpy = nat.record_dtype(self.N)
+        # sweep knobs (results do not depend on them); set by the autotuner
+        self.merge_rounds = 0
+        self.state_mode = nat.PC_STATE_AUTO
 
     # -- permutations -----------------------------------------------------
     def upload_permutations(self, perms):
@@ -81,12 +84,21 @@ class BondEngine(object):
         return out
 
     # -- sweep ------------------------------------------------------------
-    def workspace_bytes(self, num_runs):
+    def workspace_bytes(self, num_runs, state_mode=None):
         b = ctypes.c_uint64(0)
-        nat.check(self.lib.pc_sweep_workspace_size(self.N, num_runs,
+        mode = self.state_mode if state_mode is None else state_mode
+        nat.check(self.lib.pc_sweep_workspace_size(self.N, num_runs, mode,
                                                    ctypes.byref(b)),
                   'pc_sweep_workspace_size')
         return int(b.value)
+
+    def occupancy(self, state_mode=None):
+        """Resident sweep workgroups per CU (HIP occupancy calculator)."""
+        v = ctypes.c_int32(0)
+        mode = self.state_mode if state_mode is None else state_mode
+        nat.check(self.lib.pc_sweep_occupancy(self.N, mode, ctypes.byref(v)),
+                  'pc_sweep_occupancy')
+        return int(v.value)
 
     def sweep(self, d_perm, records=None, span_step=None, workspace=None):
         torch = self.torch
@@ -99,12 +111,15 @@ class BondEngine(object):
         wb = self.workspace_bytes(R)
         if wb and workspace is None:
             workspace = torch.empty((wb,), dtype=torch.uint8, device=self.device)
+        elif workspace is not None:
+            wb = int(workspace.numel())
         st = nat.stream_handle(torch)
         nat.check(self.lib.pc_sweep(
             self.N, self.M, nat.ptr(self.d_edges), nat.ptr(self.d_sides),
             self.span_first, nat.ptr(d_perm), max(self.M, 1), R,
             nat.ptr(records), max(self.M, 1), nat.ptr(span_step),
-            nat.ptr(workspace), wb, st), 'pc_sweep')
+            nat.ptr(workspace), wb, self.merge_rounds, self.state_mode, st),
+            'pc_sweep')
         return SweepResult(self, R, records, span_step, d_perm)
 
     def chunk_scan(self, res, out=None):

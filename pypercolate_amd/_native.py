@@ -21,6 +21,9 @@ PC_EHIP = -3
 PC_EOVERFLOW = -4
 
 PC_NO_SPAN = 0xFFFFFFFF
+PC_STATE_AUTO = 0
+PC_STATE_LDS = 1
+PC_STATE_HBM = 2
 PC_CHUNK = 64
 PC_STATE_WORDS = 8
 PC_COLUMNS = 7
@@ -37,9 +40,11 @@ SIGNATURES = {
     'pc_last_error': (ctypes.c_char_p, []),
     'pc_record_bytes': (ctypes.c_int, [_u32]),
     'pc_permute_philox': (ctypes.c_int, [_u64, _u64, _u32, _u32, _vp, _i64, _vp]),
-    'pc_sweep_workspace_size': (ctypes.c_int, [_u32, _u32, ctypes.POINTER(_u64)]),
+    'pc_sweep_workspace_size': (ctypes.c_int, [_u32, _u32, _u32,
+                                               ctypes.POINTER(_u64)]),
     'pc_sweep': (ctypes.c_int, [_u32, _u32, _vp, _vp, _i32, _vp, _i64, _u32,
-                                _vp, _i64, _vp, _vp, _u64, _vp]),
+                                _vp, _i64, _vp, _vp, _u64, _u32, _u32, _vp]),
+    'pc_sweep_occupancy': (ctypes.c_int, [_u32, _u32, ctypes.POINTER(_i32)]),
     'pc_chunk_scan': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp]),
     'pc_microcanonical_sums': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp,
                                               _vp, _vp, _vp, _vp]),

@@ -1337,12 +1337,13 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
 }
 
 extern "C" int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
-                                       uint64_t *bytes)
+                                       uint32_t state_mode, uint64_t *bytes)
 {
     g_last_error[0] = 0;
     if (!bytes)
         return pc_fail(PC_EINVAL, "pc_sweep_workspace_size: bytes is NULL");
-    if (num_nodes <= kLdsMaxNodes) {
+    if (state_mode == PC_STATE_LDS ||
+        (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes)) {
         *bytes = 0;
     } else {
         const uint64_t slots = num_runs < kGlobalSlots ? num_runs : kGlobalSlots;
@@ -1357,8 +1358,19 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
                         int64_t perm_stride, uint32_t num_runs, void *d_records,
                         int64_t rec_stride, uint32_t *d_span_step,
                         void *d_workspace, uint64_t workspace_bytes,
+                        uint32_t merge_rounds, uint32_t state_mode,
                         void *stream)
 {
+    const uint32_t rounds = merge_rounds ? merge_rounds : g_merge_rounds;
+    if (rounds > 64)
+        return pc_fail(PC_EINVAL, "pc_sweep: merge_rounds > 64");
+    if (state_mode > PC_STATE_HBM)
+        return pc_fail(PC_EINVAL, "pc_sweep: unknown state_mode %u", state_mode);
+    if (state_mode == PC_STATE_LDS && num_nodes > kLdsMaxNodes)
+        return pc_fail(PC_EINVAL, "pc_sweep: LDS state needs num_nodes <= %u",
+                       kLdsMaxNodes);
+    const bool use_lds = state_mode == PC_STATE_LDS ||
+                         (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes);
     g_last_error[0] = 0;
     if (num_runs == 0)
         return PC_OK;
@@ -1374,18 +1386,18 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     const int2 *edges = (const int2 *)d_edges;
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
-    if (num_nodes <= kLdsMaxNodes) {
+    if (use_lds) {
         const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
                            (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
         sweep_lds_kernel<uint32_t><<<num_runs, 64, lds, s>>>(
             num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
             d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
-            d_span_step, g_merge_rounds);
+            d_span_step, rounds);
         PC_LAUNCH_CHECK("sweep_lds_kernel");
         return PC_OK;
     }
     uint64_t need = 0;
-    pc_sweep_workspace_size(num_nodes, num_runs, &need);
+    pc_sweep_workspace_size(num_nodes, num_runs, state_mode, &need);
     if (!d_workspace || workspace_bytes < (uint64_t)num_nodes * 4u)
         return pc_fail(PC_ENOMEM,
                        "pc_sweep: workspace of %llu bytes too small (need %llu)",
@@ -1400,12 +1412,12 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         sweep_global_kernel<uint32_t><<<(uint32_t)slots, 64, 0, s>>>(
             num_nodes, num_edges, num_runs, edges, d_node_sides,
             span_on_first_merge, d_perm, perm_stride, (uint32_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace, g_merge_rounds);
+            rec_stride, d_span_step, (uint32_t *)d_workspace, rounds);
     } else {
         sweep_global_kernel<uint64_t><<<(uint32_t)slots, 64, 0, s>>>(
             num_nodes, num_edges, num_runs, edges, d_node_sides,
             span_on_first_merge, d_perm, perm_stride, (uint64_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace, g_merge_rounds);
+            rec_stride, d_span_step, (uint32_t *)d_workspace, rounds);
     }
     PC_LAUNCH_CHECK("sweep_global_kernel");
     return PC_OK;
@@ -1686,3 +1698,27 @@ extern "C" int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
     PC_LAUNCH_CHECK("micro_moments_kernel");
     return PC_OK;
 }
+
+extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
+                                  int32_t *blocks_per_cu)
+{
+    g_last_error[0] = 0;
+    if (!blocks_per_cu)
+        return pc_fail(PC_EINVAL, "pc_sweep_occupancy: NULL output");
+    int blocks = 0;
+    if (state_mode == PC_STATE_LDS ||
+        (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes)) {
+        if (num_nodes > kLdsMaxNodes)
+            return pc_fail(PC_EINVAL, "pc_sweep_occupancy: too many nodes for LDS");
+        const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
+                           (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
+        PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, sweep_lds_kernel<uint32_t>, 64, lds));
+    } else {
+        PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, sweep_global_kernel<uint32_t>, 64, 0));
+    }
+    *blocks_per_cu = blocks;
+    return PC_OK;
+}
+
