@@ -180,6 +180,29 @@ int pc_canonical_contract(uint32_t num_steps, uint32_t cols, const void *d_x,
                           int32_t precision, void *d_workspace,
                           uint64_t workspace_bytes, void *stream);
 
+/*
+ * Per-run observables over a window of steps as FP64 columns, the inputs of
+ * the per-run canonical statistics (percolate/hpc.py:443-515):
+ * d_out[(s - g_lo) * (R * C) + r * C + c] for s in [g_lo, g_lo + g_len),
+ * C = 7 ([has_spanning_cluster,] max_cluster_size, moments[0..4]) with
+ * spanning, 6 without.  Moments are their uint64 values as in
+ * bond_microcanonical_statistics.
+ */
+int pc_window_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
+                     const void *d_records, int64_t rec_stride,
+                     const uint32_t *d_span_step, const uint64_t *d_chunk_state,
+                     int64_t g_lo, int64_t g_len, int32_t spanning,
+                     double *d_out, void *stream);
+
+/*
+ * (mean, M2) over runs of per-run canonical statistics
+ * d_q[p * (R * cols) + r * cols + c]: the number_of_runs / *_mean / *_m2 of
+ * reduce(bond_reduce, map(bond_initialize_canonical_averages, ...))
+ * (percolate/hpc.py:561-702), two-pass in a fixed order.
+ */
+int pc_run_moments(const double *d_q, uint32_t num_p, uint32_t num_runs,
+                   uint32_t cols, double *d_mean, double *d_m2, void *stream);
+
 /* FP64 -> FP32 conversion of n elements (inputs of the FP32 precisions). */
 int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
                           void *stream);

@@ -1187,6 +1187,88 @@ __global__ __launch_bounds__(256) void f64_to_f32_kernel(
         out[i] = (float)in[i];
 }
 
+
+// ---------------------------------------------------------------------------
+// Per-run observables as FP64 columns over a window of steps, the input of
+// the per-run canonical statistics (percolate/hpc.py:496-512):
+// out[(s - g_lo) * (R * C) + r * C + c] for steps s in [g_lo, g_lo + g_len),
+// columns [has_spanning_cluster if spanning,] max_cluster_size,
+// moments[0..4] (moments as their uint64 values, like the reference's
+// structured array).  Block (chunk of 64 steps, run), lane = step.
+// ---------------------------------------------------------------------------
+template <typename RecT, bool WIDE>
+__global__ __launch_bounds__(64) void window_traces_kernel(
+    uint32_t N, uint32_t M, uint32_t R, const RecT *__restrict__ records,
+    int64_t rec_stride, const uint32_t *__restrict__ span_step,
+    const uint64_t *__restrict__ chunk_state, int64_t g_lo, int64_t g_len,
+    int spanning, double *__restrict__ out)
+{
+    using MT = typename Wide<WIDE>::T;
+    const uint32_t r = blockIdx.y, lane = threadIdx.x;
+    const uint32_t C = (M + 63u) / 64u;
+    // chunk index of this block within the steps 1..M (state after c*64)
+    const int64_t first_chunk = g_lo > 0 ? (g_lo - 1) / 64 : 0;
+    const int64_t c = first_chunk + blockIdx.x;
+    const int cols = spanning ? 7 : 6;
+    const int64_t width = (int64_t)R * cols;
+    if (c >= (int64_t)C) {
+        // steps past M cannot exist; only step 0 needs the no-edge case
+        return;
+    }
+    const int64_t q = c * 64 + lane; // record index, step q + 1
+    const int64_t step = q + 1;
+    const bool valid = q < (int64_t)M;
+    const RunState<WIDE> st0 = load_state<WIDE>(
+        chunk_state + ((size_t)r * C + (size_t)c) * PC_STATE_WORDS);
+    const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+    const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
+    MT x[7];
+    observables<WIDE>(st, N, (uint64_t)step >= span_step[r], x);
+    if (valid && step >= g_lo && step < g_lo + g_len) {
+        double *row = out + (step - g_lo) * width + (int64_t)r * cols;
+        int k = 0;
+        if (spanning)
+            row[k++] = (double)(uint64_t)x[1];
+        row[k++] = (double)(uint64_t)x[0];
+        for (int m = 0; m < 5; ++m)
+            row[k++] = (double)(uint64_t)x[2 + m];
+    }
+    if (c == first_chunk && lane == 0 && g_lo == 0) {
+        // step 0: max 1, no spanning, moments N - 1
+        double *row = out + (int64_t)r * cols;
+        int k = 0;
+        if (spanning)
+            row[k++] = 0.0;
+        row[k++] = 1.0;
+        for (int m = 0; m < 5; ++m)
+            row[k++] = (double)((uint64_t)N - 1u);
+    }
+}
+
+// Mean and sum of squared deviations over runs of q[p][r * cols + c]
+// (two passes, fixed order): the (n, mean, M2) of percolate/hpc.py:561-702.
+__global__ __launch_bounds__(256) void run_moments_kernel(
+    const double *__restrict__ q, uint32_t P, uint32_t R, uint32_t cols,
+    double *__restrict__ mean, double *__restrict__ m2)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)P * cols)
+        return;
+    const uint64_t p = i / cols, c = i % cols;
+    const double *row = q + p * (uint64_t)R * cols + c;
+    double s = 0.0;
+    for (uint32_t r = 0; r < R; ++r)
+        s += row[(uint64_t)r * cols];
+    const double mu = s / (double)R;
+    double d2 = 0.0;
+    for (uint32_t r = 0; r < R; ++r) {
+        const double d = row[(uint64_t)r * cols] - mu;
+        d2 = fma(d, d, d2);
+    }
+    mean[i] = mu;
+    m2[i] = d2;
+}
+
 // ---------------------------------------------------------------------------
 // Microcanonical mean and sample standard deviation from the exact sums:
 // mean = S1 / R, std = sqrt((R S2 - S1^2) / (R (R - 1))), with the numerator
@@ -1592,6 +1674,19 @@ extern "C" int pc_binomial_weights(uint32_t n, uint32_t num_p,
     return PC_OK;
 }
 
+// K-slices of the MFMA contraction: enough tiles x slices to fill the GPU
+// with about 16k wavefronts, at most PC_CONTRACT_SLICES.
+static uint32_t contract_slices(uint32_t num_p, uint32_t cols)
+{
+    const uint64_t tiles = (uint64_t)((num_p + 15u) / 16u) * ((cols + 15u) / 16u);
+    uint64_t sl = (16384u + tiles - 1u) / (tiles ? tiles : 1u);
+    if (sl < 1u)
+        sl = 1u;
+    if (sl > PC_CONTRACT_SLICES)
+        sl = PC_CONTRACT_SLICES;
+    return (uint32_t)sl;
+}
+
 extern "C" int pc_contract_workspace_size(uint32_t num_p, uint32_t cols,
                                           int32_t precision, uint64_t *bytes)
 {
@@ -1600,7 +1695,8 @@ extern "C" int pc_contract_workspace_size(uint32_t num_p, uint32_t cols,
         return pc_fail(PC_EINVAL, "pc_contract_workspace_size: bytes is NULL");
     *bytes = precision == PC_PREC_FP64
                  ? 0u
-                 : (uint64_t)PC_CONTRACT_SLICES * num_p * cols * sizeof(double);
+                 : (uint64_t)contract_slices(num_p, cols) * num_p * cols *
+                       sizeof(double);
     return PC_OK;
 }
 
@@ -1615,8 +1711,9 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
     g_last_error[0] = 0;
     if (num_p == 0 || cols == 0)
         return PC_OK;
-    if (cols > CONTRACT_MAX_COLS)
-        return pc_fail(PC_EOVERFLOW, "pc_canonical_contract: more than %d columns",
+    if (precision == PC_PREC_FP64 && cols > CONTRACT_MAX_COLS)
+        return pc_fail(PC_EOVERFLOW,
+                       "pc_canonical_contract: FP64 vector form takes at most %d columns",
                        CONTRACT_MAX_COLS);
     if (!d_x || !d_start || !d_len || !d_w || !d_out || x_stride < (int64_t)cols)
         return pc_fail(PC_EINVAL, "pc_canonical_contract: bad buffer");
@@ -1641,20 +1738,21 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
         return pc_fail(PC_ENOMEM, "pc_canonical_contract: workspace needs %llu bytes",
                        (unsigned long long)need);
     const TileCtx t{num_p, cols, x_stride, kmax, d_start, d_len};
-    const dim3 grid((num_p + 15u) / 16u, (cols + 15u) / 16u, PC_CONTRACT_SLICES);
+    const uint32_t slices = contract_slices(num_p, cols);
+    const dim3 grid((num_p + 15u) / 16u, (cols + 15u) / 16u, slices);
     double *partial = (double *)d_workspace;
     switch (precision) {
     case PC_PREC_FP64_MFMA:
         contract_mfma_kernel<PC_PREC_FP64_MFMA><<<grid, 64, 0, s>>>(
-            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+            d_x, num_steps, d_w, t, slices, partial);
         break;
     case PC_PREC_FP32_MFMA:
         contract_mfma_kernel<PC_PREC_FP32_MFMA><<<grid, 64, 0, s>>>(
-            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+            d_x, num_steps, d_w, t, slices, partial);
         break;
     case PC_PREC_BF16X3_MFMA:
         contract_mfma_kernel<PC_PREC_BF16X3_MFMA><<<grid, 64, 0, s>>>(
-            d_x, num_steps, d_w, t, PC_CONTRACT_SLICES, partial);
+            d_x, num_steps, d_w, t, slices, partial);
         break;
     default:
         return pc_fail(PC_EINVAL, "pc_canonical_contract: unknown precision %d",
@@ -1663,7 +1761,7 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
     PC_LAUNCH_CHECK("contract_mfma_kernel");
     const int64_t n = (int64_t)num_p * cols;
     contract_reduce_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(
-        partial, PC_CONTRACT_SLICES, n, d_out);
+        partial, slices, n, d_out);
     PC_LAUNCH_CHECK("contract_reduce_kernel");
     return PC_OK;
 }
@@ -1722,3 +1820,57 @@ extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
     return PC_OK;
 }
 
+extern "C" int pc_window_traces(uint32_t num_nodes, uint32_t num_edges,
+                                uint32_t num_runs, const void *d_records,
+                                int64_t rec_stride, const uint32_t *d_span_step,
+                                const uint64_t *d_chunk_state, int64_t g_lo,
+                                int64_t g_len, int32_t spanning, double *d_out,
+                                void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0 || g_len <= 0)
+        return PC_OK;
+    if (num_nodes == 0 || num_nodes > kSumsMaxNodes)
+        return pc_fail(PC_EOVERFLOW, "pc_window_traces: num_nodes outside 1..2^21");
+    if (g_lo < 0 || g_lo + g_len > (int64_t)num_edges + 1)
+        return pc_fail(PC_EINVAL, "pc_window_traces: window outside 0..num_edges");
+    if (num_edges == 0)
+        return pc_fail(PC_EINVAL, "pc_window_traces: graph without edges");
+    if (!d_records || !d_span_step || !d_chunk_state || !d_out)
+        return pc_fail(PC_EINVAL, "pc_window_traces: NULL buffer");
+    const int64_t first_chunk = g_lo > 0 ? (g_lo - 1) / 64 : 0;
+    const int64_t last_step = g_lo + g_len - 1;
+    const int64_t last_chunk = last_step > 0 ? (last_step - 1) / 64 : 0;
+    const dim3 grid((uint32_t)(last_chunk - first_chunk + 1), num_runs);
+    hipStream_t s = as_stream(stream);
+    if (num_nodes <= kLdsMaxNodes)
+        window_traces_kernel<uint32_t, false><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, (const uint32_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, g_lo, g_len, spanning, d_out);
+    else if (num_nodes <= kU32RecordMaxNodes)
+        window_traces_kernel<uint32_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, (const uint32_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, g_lo, g_len, spanning, d_out);
+    else
+        window_traces_kernel<uint64_t, true><<<grid, 64, 0, s>>>(
+            num_nodes, num_edges, num_runs, (const uint64_t *)d_records,
+            rec_stride, d_span_step, d_chunk_state, g_lo, g_len, spanning, d_out);
+    PC_LAUNCH_CHECK("window_traces_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_run_moments(const double *d_q, uint32_t num_p,
+                              uint32_t num_runs, uint32_t cols, double *d_mean,
+                              double *d_m2, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0 || cols == 0)
+        return PC_OK;
+    if (num_runs == 0 || !d_q || !d_mean || !d_m2)
+        return pc_fail(PC_EINVAL, "pc_run_moments: bad arguments");
+    const uint64_t n = (uint64_t)num_p * cols;
+    run_moments_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+        d_q, num_p, num_runs, cols, d_mean, d_m2);
+    PC_LAUNCH_CHECK("run_moments_kernel");
+    return PC_OK;
+}

@@ -351,3 +351,104 @@ def finalize_canonical_averages(
     _transform('max_cluster_size', 'percolation_strength', normalize=True)
     _transform('moments', normalize=True, transpose=True)
     return ret
+
+
+def bond_canonical_averages_batch(
+    perc_graph, num_nodes, num_edges, ps, seeds=None, num_runs=None,
+    philox_seed=0, spanning_cluster=True, auxiliary_node_attributes=None,
+    auxiliary_edge_attributes=None, spanning_sides=None, run_chunk=1024,
+    **kwargs
+):
+    """Canonical averages over many runs in batched GPU launches: the result
+    of the reference's jug study for one system size,
+    ``reduce(bond_reduce, [bond_initialize_canonical_averages(
+    [bond_canonical_statistics(bond_microcanonical_statistics(seed), f_p)
+    for p in ps]) for seed in seeds])`` (percolate/share/jugfile.py:57-135,
+    percolate/hpc.py:443-702).
+
+    With ``seeds`` the runs use numpy RandomState(seed).permutation(M)
+    (bit-exact traces); otherwise ``num_runs`` runs draw device permutations
+    keyed by ``philox_seed``.  Per chunk of runs: sweep, per-run observables
+    over the union of the convolution windows, one FP64 matrix-core
+    contraction of all runs' columns with the binomial weights, and the
+    per-(p, field) mean and M2 over the chunk's runs; chunks merge with the
+    Chan formula.  Returns an array of canonical_averages_dtype (one row per
+    p), ready for finalize_canonical_averages.
+    """
+    import torch
+    from . import _engine as E
+    from ._engine import BondEngine, ContractPlan, Weights
+
+    _check_sides(spanning_cluster, spanning_sides)
+    ga = _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
+                       auxiliary_edge_attributes, spanning_sides)
+    if ga.num_nodes != num_nodes or ga.num_edges != num_edges:
+        raise ValueError('num_nodes / num_edges do not match perc_graph')
+    if ga.num_edges == 0:
+        raise ValueError('the percolation graph has no edges')
+    ps = np.atleast_1d(np.asarray(ps, dtype=np.float64))
+    if seeds is not None:
+        seeds = list(seeds)
+        total = len(seeds)
+    else:
+        total = int(num_runs)
+    if total < 1:
+        raise ValueError('at least one run is needed')
+    cols = 7 if spanning_cluster else 6
+    eng = BondEngine(ga)
+    M = eng.M
+    wt = E.binomial_weights(M, ps)
+    g_lo = int(wt.host_start.min())
+    g_hi = int((wt.host_start + wt.host_length).max())
+    g_len = g_hi - g_lo
+    shifted = Weights(M, ps, wt.start - g_lo, wt.length, wt.kmax, wt.w)
+    shifted.host_start = wt.host_start - g_lo
+    shifted.host_length = wt.host_length
+    chunk = max(1, min(int(run_chunk), (1 << 30) // (g_len * cols * 8)))
+
+    n_acc = None
+    for first in range(0, total, chunk):
+        rc = min(chunk, total - first)
+        if seeds is not None:
+            perms = np.stack([np.random.RandomState(seed=s).permutation(M)
+                              for s in seeds[first:first + rc]])
+            d_perm = eng.upload_permutations(perms)
+        else:
+            d_perm = eng.device_permutations(philox_seed, rc, first_run=first)
+        res = eng.sweep(d_perm)
+        x = torch.empty((g_len, rc * cols), dtype=torch.float64,
+                        device=eng.device)
+        lib = eng.lib
+        from . import _native as nat
+        nat.check(lib.pc_window_traces(
+            eng.N, M, rc, nat.ptr(res.records), max(M, 1),
+            nat.ptr(res.span_step), nat.ptr(res.chunk_state), g_lo, g_len,
+            1 if spanning_cluster else 0, nat.ptr(x),
+            nat.stream_handle(torch)), 'pc_window_traces')
+        q = ContractPlan(x, shifted, nat.PC_PREC_FP64_MFMA).run()
+        mean = torch.empty((ps.size, cols), dtype=torch.float64, device=eng.device)
+        m2 = torch.empty_like(mean)
+        nat.check(lib.pc_run_moments(nat.ptr(q), ps.size, rc, cols,
+                                     nat.ptr(mean), nat.ptr(m2),
+                                     nat.stream_handle(torch)), 'pc_run_moments')
+        mean = mean.cpu().numpy()
+        m2 = m2.cpu().numpy()
+        if n_acc is None:
+            n_acc, mean_acc, m2_acc = float(rc), mean, m2
+        else:
+            n_acc, mean_acc, m2_acc = _online_variance(
+                (n_acc, mean_acc, m2_acc), (float(rc), mean, m2))
+        del x, q, res, d_perm
+
+    ret = np.empty(ps.size, dtype=canonical_averages_dtype(spanning_cluster))
+    ret['number_of_runs'] = total
+    off = 0
+    if spanning_cluster:
+        ret['percolation_probability_mean'] = mean_acc[:, 0]
+        ret['percolation_probability_m2'] = m2_acc[:, 0]
+        off = 1
+    ret['max_cluster_size_mean'] = mean_acc[:, off]
+    ret['max_cluster_size_m2'] = m2_acc[:, off]
+    ret['moments_mean'] = mean_acc[:, off + 1:off + 6]
+    ret['moments_m2'] = m2_acc[:, off + 1:off + 6]
+    return ret

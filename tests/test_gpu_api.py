@@ -265,3 +265,37 @@ def test_bond_pipeline_reduce_finalize(gpu, percolation_graph):
                                 scale=fin['percolation_strength_std'][ok] / np.sqrt(len(seeds)),
                                 df=len(seeds) - 1)
         np.testing.assert_allclose(cdf, a, rtol=1e-4)
+
+
+@pytest.mark.parametrize('L,spanning', [(3, True), (16, True), (16, False)])
+def test_bond_canonical_averages_batch(gpu, L, spanning):
+    """The jug study's map-reduce on the device equals the oracle's per-run
+    canonical statistics reduced over runs (means rtol 1e-11, M2 rtol 1e-9
+    of the largest M2 of the field)."""
+    graph = percolate.spanning_2d_grid(L) if spanning else nx.grid_2d_graph(L, L)
+    pg = percolate.percolation_graph(graph, spanning_cluster=spanning)
+    ps = np.linspace(0.3, 0.7, 9)
+    seeds = list(range(100, 140))
+    res = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds, run_chunk=16, **pg)
+    og = oracle.percolation_graph_arrays(graph, spanning)
+    tr = oracle.bond_runs(og, oracle.randomstate_permutations(seeds, og['num_edges']))
+    factors = [oracle.binomial_pmf(og['num_edges'], p) for p in ps]
+    stats = []
+    for r in range(len(seeds)):
+        one = {k: v[r] for k, v in tr.items()}
+        if not spanning:
+            one.pop('has_spanning_cluster')
+        stats.append(oracle.initialize_canonical_averages(
+            [oracle.canonical_statistics(one, f) for f in factors]))
+    ref = functools.reduce(oracle.bond_reduce, stats)
+    assert (res['number_of_runs'] == len(seeds)).all()
+    for key in ref:
+        if key == 'number_of_runs':
+            continue
+        if key.endswith('_m2'):
+            scale = np.abs(ref[key]).max(axis=0, keepdims=True) + 1e-300
+            np.testing.assert_allclose(res[key] / scale, ref[key] / scale,
+                                       rtol=0, atol=1e-9, err_msg=key)
+        else:
+            np.testing.assert_allclose(res[key], ref[key], rtol=1e-11,
+                                       atol=1e-13, err_msg=key)
