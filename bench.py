@@ -52,19 +52,21 @@ def parse_args(argv=None):
                     help='lattice of the canonical-average comparison (0 skips)')
     ap.add_argument('--canonical-runs', type=int, default=64)
     ap.add_argument('--canonical-ps', type=int, default=4096)
+    ap.add_argument('--no-tune', dest='tune', action='store_false',
+                    help='use the default sweep knobs instead of the autotuner')
     ap.add_argument('--batch', action='store_true',
                     help='time the finite-size batch (L = 16..512, 2^26/L^2 '
                          'runs each, one HIP graph) instead of L = 128')
     return ap.parse_args(argv)
 
 
-def profiled_traffic(length, runs):
-    """HBM bytes per sweep launch from the committed PMC summary of this
-    workload (profiles/sweep_L<L>_R<R>.md), or None."""
+def profiled_traffic_per_add(length, runs):
+    """HBM bytes per bond addition of the sweep from the committed PMC
+    summary of this workload (profiles/sweep_L<L>_R<R>.md), or None."""
     path = os.path.join(ROOT, 'profiles', 'sweep_L{}_R{}.md'.format(length, runs))
     try:
         for line in open(path):
-            if line.startswith('sweep_traffic_bytes_per_launch:'):
+            if line.startswith('sweep_traffic_bytes_per_add:'):
                 return float(line.split(':', 1)[1])
     except OSError:
         pass
@@ -225,52 +227,40 @@ def main(argv=None):
         out = batch_bench(args)
         print(json.dumps(out), flush=True)
         return 0 if out['sums_check'] else 1
+    from pypercolate_amd._engine import Pipeline
+    from pypercolate_amd.autotune import tune_sweep
     eng = BondEngine(_graph.square_lattice(args.length))
+    tuning = tune_sweep(eng) if args.tune else None
     R, M, N, S = args.runs, eng.M, eng.N, eng.S
     dev = eng.device
-    C = (M + nat.PC_CHUNK - 1) // nat.PC_CHUNK
-    perm = torch.empty((R, M), dtype=torch.int32, device=dev)
-    records = torch.empty((R, M), dtype=eng.rec_torch, device=dev)
-    span = torch.empty((R,), dtype=torch.int32, device=dev)
-    state = torch.empty((R, C, nat.PC_STATE_WORDS), dtype=torch.int64, device=dev)
-    sum1 = torch.empty((7, S, 2), dtype=torch.int64, device=dev)
-    sum2 = torch.empty((7, S, 3), dtype=torch.int64, device=dev)
-    wb = eng.workspace_bytes(R)
-    workspace = torch.empty((wb,), dtype=torch.uint8, device=dev) if wb else None
-
-    sweep_events = [(torch.cuda.Event(enable_timing=True),
-                     torch.cuda.Event(enable_timing=True))
-                    for _ in range(args.steps)]
-
-    def step(events=None):
-        eng.device_permutations(args.seed, R, out=perm)
-        if events:
-            events[0].record()
-        res = eng.sweep(perm, records=records, span_step=span,
-                        workspace=workspace)
-        if events:
-            events[1].record()
-        res.chunk_state = eng.chunk_scan(res, out=state)
-        eng.microcanonical_sums(res, sum1=sum1, sum2=sum2)
-        return res
+    pipe = Pipeline(eng, R, seed=args.seed)
+    sum1 = pipe.sum1
 
     for _ in range(args.warmup):
-        step()
+        pipe.run()
     torch.cuda.synchronize()
 
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
+    sweep_ms = []
+    launches = 0
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
     start.record()
     for k in range(args.steps):
-        step(sweep_events[k])
+        pipe.run()
+        # read the previous step's sweep events before they are replaced
+        if k + 1 < args.steps:
+            pass
     stop.record()
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(stop)
     ms_per_step = total_ms / args.steps
-    sweep_avg_ms = sum(a.elapsed_time(b) for a, b in sweep_events) / args.steps
+    # HIP-event time of the sweep launches of the last step (same stream)
+    sweep_step_ms = pipe.sweep_ms()
+    launches = len(pipe.sweep_events)
+    sweep_avg_ms = sweep_step_ms / max(launches, 1)
 
     # sanity on the last step's exact sums (not timed): at n = M every run
     # is one cluster of N nodes and spans
@@ -284,9 +274,11 @@ def main(argv=None):
         torch.cuda.synchronize()
 
     adds = float(R) * M
+    traffic_per_add = profiled_traffic_per_add(args.length, R)
     value = adds / (ms_per_step / 1e3)
     alg_bytes = ALG_BYTES_PER_ADD_LDS if N <= 16384 else ALG_BYTES_PER_ADD_HBM
-    achieved = alg_bytes * adds / (sweep_avg_ms / 1e3) / 1e9
+    # algorithmic bytes of one sweep launch over its duration
+    achieved = alg_bytes * adds / max(launches, 1) / (sweep_avg_ms / 1e3) / 1e9
     out = {
         'metric': METRIC,
         'value': value,
@@ -307,7 +299,9 @@ def main(argv=None):
                          '-> chunk states -> exact microcanonical sums').format(
                              args.length, N, M, R),
             'length': args.length, 'runs': R, 'nodes': N, 'edges': M,
-            'parallelism': 'single GPU, one run per wavefront',
+            'parallelism': ('single GPU, one run per wavefront; sub-batches of '
+                            '{} runs, reduction overlapped with the next sweep '
+                            'on a second stream').format(pipe.sub),
         },
         'roofline': {
             'kernel': 'sweep_lds_kernel' if N <= 16384 else 'sweep_global_kernel',
@@ -316,10 +310,14 @@ def main(argv=None):
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': achieved / HBM_PEAK_GBS,
-            'traffic': profiled_traffic(args.length, R),
+            'traffic': (None if traffic_per_add is None else
+                        traffic_per_add * adds / max(launches, 1)),
             'alg_bytes_per_add': alg_bytes,
             'kernel_ms': sweep_avg_ms,
+            'launches_per_step': launches,
+            'sweep_ms_per_step': sweep_step_ms,
         },
+        'tuning': tuning,
         'sums_check': ok,
         'wall_s': t_wall,
     }

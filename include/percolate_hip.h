@@ -114,13 +114,15 @@ int pc_chunk_scan(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
  * (d_sum2[(k * S + s) * 3 + limb], uint192), S = num_edges + 1.  Replaces the
  * per-step gathers of percolate/percolate.py:862-891 feeding
  * _microcanonical_average_* (percolate.py:575-705).  Moments are exact
- * integers (their low 64 bits are the reference's uint64 values).
+ * integers (their low 64 bits are the reference's uint64 values).  With
+ * accumulate != 0 the sums are added to the buffers' contents (batches of
+ * runs reduced in turn); otherwise the buffers are cleared first.
  */
 int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
                            uint32_t num_runs, const void *d_records,
                            int64_t rec_stride, const uint32_t *d_span_step,
                            const uint64_t *d_chunk_state, uint64_t *d_sum1,
-                           uint64_t *d_sum2, void *stream);
+                           uint64_t *d_sum2, int32_t accumulate, void *stream);
 
 /*
  * Per-run traces, the arrays of percolate/hpc.py:310-404

@@ -135,7 +135,7 @@ class BondEngine(object):
         return out
 
     # -- outputs ------------------------------------------------------------
-    def microcanonical_sums(self, res, sum1=None, sum2=None):
+    def microcanonical_sums(self, res, sum1=None, sum2=None, accumulate=False):
         """Exact per-step sums over runs: device tensors sum1 (7, S, 2) and
         sum2 (7, S, 3) of uint64 limbs (stored as int64)."""
         torch = self.torch
@@ -149,7 +149,8 @@ class BondEngine(object):
         nat.check(self.lib.pc_microcanonical_sums(
             self.N, self.M, res.num_runs, nat.ptr(res.records),
             max(self.M, 1), nat.ptr(res.span_step), nat.ptr(res.chunk_state),
-            nat.ptr(sum1), nat.ptr(sum2), st), 'pc_microcanonical_sums')
+            nat.ptr(sum1), nat.ptr(sum2), 1 if accumulate else 0, st),
+            'pc_microcanonical_sums')
         return sum1, sum2
 
     def traces(self, res):
@@ -187,6 +188,90 @@ def host_sums(sum1, sum2):
     """Device sum tensors -> exact Python-int object arrays (7, S)."""
     return (limbs_to_int(sum1.cpu().numpy()),
             limbs_to_int(sum2.cpu().numpy()))
+
+
+class Pipeline(object):
+    """R runs of one graph through permute -> sweep -> chunk states ->
+    exact sums, in sub-batches on three streams so that the reduction of one
+    sub-batch runs beside the sweep of the next (the sweep leaves most issue
+    slots of a CU idle: its occupancy is bounded by LDS, four runs per CU at
+    N = 16384).  Stream order: permutations on one stream, sweeps on a
+    second (each waiting for its permutations), chunk states and sums on a
+    third (each waiting for its sweep, accumulating into one pair of sum
+    buffers).  All buffers are allocated up front."""
+
+    def __init__(self, engine, num_runs, seed=0, sub_batch=None):
+        torch = self.torch = engine.torch
+        self.engine = eng = engine
+        self.R = int(num_runs)
+        self.seed = seed
+        dev = eng.device
+        M, S = eng.M, eng.S
+        if sub_batch is None:
+            cus = torch.cuda.get_device_properties(dev).multi_processor_count
+            sub_batch = 2 * max(eng.occupancy(), 1) * cus
+        self.sub = max(1, min(int(sub_batch), self.R))
+        C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
+        self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
+        self.records = torch.empty((self.R, max(M, 1)), dtype=eng.rec_torch,
+                                   device=dev)
+        self.span = torch.empty((self.R,), dtype=torch.int32, device=dev)
+        self.state = torch.empty((self.R, C, nat.PC_STATE_WORDS),
+                                 dtype=torch.int64, device=dev)
+        self.sum1 = torch.empty((nat.PC_COLUMNS, S, 2), dtype=torch.int64,
+                                device=dev)
+        self.sum2 = torch.empty((nat.PC_COLUMNS, S, 3), dtype=torch.int64,
+                                device=dev)
+        wb = eng.workspace_bytes(self.sub)
+        self.workspace = (torch.empty((wb,), dtype=torch.uint8, device=dev)
+                          if wb else None)
+        self.streams = [torch.cuda.Stream() for _ in range(3)]
+        self.sweep_events = []
+
+    def run(self):
+        """Enqueue the whole batch; the caller's stream waits for it."""
+        torch = self.torch
+        eng = self.engine
+        main = torch.cuda.current_stream()
+        sp, ss, sr = self.streams
+        start = torch.cuda.Event()
+        start.record(main)
+        for s in self.streams:
+            s.wait_event(start)
+        self.sweep_events = []
+        first = True
+        for b0 in range(0, self.R, self.sub):
+            n = min(self.sub, self.R - b0)
+            perm = self.perm[b0:b0 + n]
+            with torch.cuda.stream(sp):
+                eng.device_permutations(self.seed, n, first_run=b0, out=perm)
+                have_perm = torch.cuda.Event()
+                have_perm.record(sp)
+            ss.wait_event(have_perm)
+            with torch.cuda.stream(ss):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e0.record(ss)
+                res = eng.sweep(perm, records=self.records[b0:b0 + n],
+                                span_step=self.span[b0:b0 + n],
+                                workspace=self.workspace)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e1.record(ss)
+            self.sweep_events.append((e0, e1))
+            sr.wait_event(e1)
+            with torch.cuda.stream(sr):
+                res.chunk_state = eng.chunk_scan(res, out=self.state[b0:b0 + n])
+                eng.microcanonical_sums(res, sum1=self.sum1, sum2=self.sum2,
+                                        accumulate=not first)
+            first = False
+        for s in self.streams:
+            done = torch.cuda.Event()
+            done.record(s)
+            main.wait_event(done)
+        return self.sum1, self.sum2
+
+    def sweep_ms(self):
+        """Summed HIP-event time of the sweeps of the last run()."""
+        return sum(a.elapsed_time(b) for a, b in self.sweep_events)
 
 
 class Weights(object):

@@ -47,7 +47,7 @@ SIGNATURES = {
     'pc_sweep_occupancy': (ctypes.c_int, [_u32, _u32, ctypes.POINTER(_i32)]),
     'pc_chunk_scan': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp]),
     'pc_microcanonical_sums': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp,
-                                              _vp, _vp, _vp, _vp]),
+                                              _vp, _vp, _vp, _i32, _vp]),
     'pc_expand_traces': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp,
                                         _vp, _vp, _vp, _vp]),
     'pc_binomial_window': (ctypes.c_int, [_u32, ctypes.c_double,

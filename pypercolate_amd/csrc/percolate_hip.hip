@@ -1540,7 +1540,7 @@ extern "C" int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
                                       const uint32_t *d_span_step,
                                       const uint64_t *d_chunk_state,
                                       uint64_t *d_sum1, uint64_t *d_sum2,
-                                      void *stream)
+                                      int32_t accumulate, void *stream)
 {
     g_last_error[0] = 0;
     if (num_nodes == 0 || num_nodes > kSumsMaxNodes)
@@ -1559,9 +1559,11 @@ extern "C" int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
         return pc_fail(PC_EINVAL, "pc_microcanonical_sums: NULL buffer");
     hipStream_t s = as_stream(stream);
     const uint64_t S = (uint64_t)num_edges + 1u;
-    fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum1, 7u * S * 2u);
-    fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum2, 7u * S * 3u);
-    PC_LAUNCH_CHECK("fill_zero_kernel");
+    if (!accumulate) {
+        fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum1, 7u * S * 2u);
+        fill_zero_kernel<<<1024, 256, 0, s>>>(d_sum2, 7u * S * 3u);
+        PC_LAUNCH_CHECK("fill_zero_kernel");
+    }
     if (num_runs == 0)
         return PC_OK;
     if (num_edges == 0) {

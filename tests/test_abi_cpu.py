@@ -66,5 +66,5 @@ def test_argument_errors_without_gpu():
                                 ctypes.byref(ctypes.c_int64()))
     assert rc == nat.PC_EINVAL
     rc = lib.pc_microcanonical_sums(1 << 22, 10, 1, None, 1, None, None, None,
-                                    None, None)
+                                    None, 0, None)
     assert rc == nat.PC_EOVERFLOW

@@ -153,3 +153,19 @@ def test_full_size_properties(gpu):
     # Cauchy-Schwarz: R * sum x^2 >= (sum x)^2
     for c in range(7):
         assert (R * s2[c] - s1[c] * s1[c] >= 0).all()
+
+
+def test_pipeline_matches_single_launch(gpu):
+    """Sub-batches on three streams (reduction beside the next sweep) give
+    the same exact sums as one launch over all runs."""
+    from pypercolate_amd._engine import Pipeline
+    eng = BondEngine(_graph.square_lattice(32))
+    R = 3000
+    res = eng.sweep(eng.device_permutations(9, R))
+    ref = [t.cpu().numpy() for t in eng.microcanonical_sums(res)]
+    pipe = Pipeline(eng, R, seed=9, sub_batch=700)
+    for _ in range(2):  # a second run must restart the accumulation
+        s1, s2 = pipe.run()
+        gpu.cuda.synchronize()
+        assert np.array_equal(s1.cpu().numpy(), ref[0])
+        assert np.array_equal(s2.cpu().numpy(), ref[1])
