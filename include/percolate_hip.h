@@ -71,6 +71,16 @@ int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
 #define PC_STATE_LDS 1  /* 16-bit codes in LDS (num_nodes <= 16384)          */
 #define PC_STATE_HBM 2  /* 32-bit codes in the HBM workspace                 */
 
+/*
+ * Bit-exact device permutations: d_perm[r * perm_stride + i] =
+ * numpy.random.RandomState(d_seeds[r]).permutation(num_edges)[i], the
+ * permutation the reference draws per run (percolate/hpc.py:195,206), from
+ * MT19937 and numpy's legacy shuffle on the GPU (one run per wavefront).
+ */
+int pc_permute_randomstate(const uint32_t *d_seeds, uint32_t num_runs,
+                           uint32_t num_edges, int32_t *d_perm,
+                           int64_t perm_stride, void *stream);
+
 /* Device workspace bytes pc_sweep needs (0 when the state is in LDS). */
 int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
                             uint32_t state_mode, uint64_t *bytes);

@@ -83,6 +83,23 @@ class BondEngine(object):
             self.M, nat.ptr(out), self.M, st), 'pc_permute_philox')
         return out
 
+    def randomstate_permutations(self, seeds, out=None):
+        """numpy RandomState(seed).permutation(M) for every seed, drawn on
+        the device (bit-identical to numpy's legacy generator)."""
+        torch = self.torch
+        seeds = np.asarray(seeds, dtype=np.int64).reshape(-1)
+        if seeds.size and (seeds.min() < 0 or seeds.max() > 0xFFFFFFFF):
+            raise ValueError('RandomState seeds must be in [0, 2**32 - 1]')
+        d_seeds = torch.from_numpy(seeds.astype(np.uint32).view(np.int32)).to(
+            self.device)
+        if out is None:
+            out = torch.empty((seeds.size, max(self.M, 1)), dtype=torch.int32,
+                              device=self.device)
+        nat.check(self.lib.pc_permute_randomstate(
+            nat.ptr(d_seeds), seeds.size, self.M, nat.ptr(out), max(self.M, 1),
+            nat.stream_handle(torch)), 'pc_permute_randomstate')
+        return out
+
     # -- sweep ------------------------------------------------------------
     def workspace_bytes(self, num_runs, state_mode=None):
         b = ctypes.c_uint64(0)

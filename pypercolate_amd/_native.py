@@ -40,6 +40,7 @@ SIGNATURES = {
     'pc_last_error': (ctypes.c_char_p, []),
     'pc_record_bytes': (ctypes.c_int, [_u32]),
     'pc_permute_philox': (ctypes.c_int, [_u64, _u64, _u32, _u32, _vp, _i64, _vp]),
+    'pc_permute_randomstate': (ctypes.c_int, [_vp, _u32, _u32, _vp, _i64, _vp]),
     'pc_sweep_workspace_size': (ctypes.c_int, [_u32, _u32, _u32,
                                                ctypes.POINTER(_u64)]),
     'pc_sweep': (ctypes.c_int, [_u32, _u32, _vp, _vp, _i32, _vp, _i64, _u32,

@@ -127,6 +127,13 @@ __device__ __forceinline__ uint32_t wave_scan_max(uint32_t x)
 
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
+// The first active lane's value, as a wave-uniform (scalar) value.
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+
 __device__ __forceinline__ uint32_t read_lane(uint32_t v, uint32_t lane)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
@@ -258,6 +265,127 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Bit-exact device permutations: numpy's legacy RandomState(seed).permutation
+// (the permutations the reference draws, percolate/hpc.py:195,206).
+// MT19937 (Matsumoto & Nishimura, 1998) seeded with init_genrand(seed), as
+// RandomState does for an integer seed; permutation(n) shuffles arange(n)
+// from the end: for i = n-1 .. 1, j is the first 32-bit draw whose low bits
+// under the smallest all-ones mask >= i are <= i, then a[i] and a[j] swap.
+// One run per wavefront: the 624-word twist is computed by all lanes in four
+// phases that respect the recurrence's order, the draws are consumed on
+// uniform (scalar) control flow and lane 0 performs the swaps, in LDS
+// (16-bit entries, n <= 65536) or in the output row.
+// ---------------------------------------------------------------------------
+#define MT_N 624
+#define MT_SHIFT 397
+#define MT_LDS_MAX 65536u
+
+__device__ __forceinline__ uint32_t mt_next_word(uint32_t cur, uint32_t nxt,
+                                                 uint32_t far)
+{
+    const uint32_t y = (cur & 0x80000000u) | (nxt & 0x7fffffffu);
+    return far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+}
+
+// new mt[k] for k in [lo, hi), reading mt[k + 397] (k < 227) or the already
+// renewed mt[k - 227] (k >= 227)
+__device__ __forceinline__ void mt_twist_range(uint32_t *mt, uint32_t lo,
+                                               uint32_t hi)
+{
+    for (uint32_t k0 = lo; k0 < hi; k0 += 64u) {
+        const uint32_t k = k0 + threadIdx.x;
+        uint32_t v = 0;
+        if (k < hi)
+            v = mt_next_word(mt[k], mt[(k + 1u) % MT_N], mt[(k + MT_SHIFT) % MT_N]);
+        if (k < hi)
+            mt[k] = v;
+    }
+}
+
+__device__ __forceinline__ void mt_twist(uint32_t *mt)
+{
+    mt_twist_range(mt, 0u, MT_N - MT_SHIFT);        // 0..226: old words
+    mt_twist_range(mt, MT_N - MT_SHIFT, 2u * (MT_N - MT_SHIFT)); // 227..453
+    mt_twist_range(mt, 2u * (MT_N - MT_SHIFT), MT_N - 1u);      // 454..622
+    mt_twist_range(mt, MT_N - 1u, MT_N);            // 623: renewed mt[0], mt[396]
+}
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
+{
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+template <bool LDS_ARRAY>
+__global__ __launch_bounds__(64) void permute_randomstate_kernel(
+    const uint32_t *__restrict__ seeds, uint32_t n, int32_t *__restrict__ perm,
+    int64_t stride)
+{
+    __shared__ uint32_t mt[MT_N];
+    extern __shared__ uint16_t arr16[];
+    const uint32_t lane = threadIdx.x;
+    int32_t *out = perm + (int64_t)blockIdx.x * stride;
+    if (lane == 0) {
+        uint32_t x = seeds[blockIdx.x];
+        mt[0] = x;
+        for (uint32_t i = 1; i < MT_N; ++i) {
+            x = 1812433253u * (x ^ (x >> 30)) + i;
+            mt[i] = x;
+        }
+    }
+    for (uint32_t i = lane; i < n; i += 64u) {
+        if (LDS_ARRAY)
+            arr16[i] = (uint16_t)i;
+        else
+            out[i] = (int32_t)i;
+    }
+    __syncthreads();
+
+    uint32_t pos = MT_N;
+    uint32_t i = n ? n - 1u : 0u;
+    while (i >= 1u) {
+        uint32_t mask = i;
+        mask |= mask >> 1;
+        mask |= mask >> 2;
+        mask |= mask >> 4;
+        mask |= mask >> 8;
+        mask |= mask >> 16;
+        uint32_t v;
+        do {
+            if (pos == MT_N) {
+                __syncthreads();
+                mt_twist(mt);
+                __syncthreads();
+                pos = 0;
+            }
+            v = mt_temper(uniform(mt[pos])) & mask;
+            ++pos;
+        } while (v > i);
+        if (lane == 0) {
+            if (LDS_ARRAY) {
+                const uint16_t a = arr16[i], b = arr16[v];
+                arr16[i] = b;
+                arr16[v] = a;
+            } else {
+                const int32_t a = out[i], b = out[v];
+                out[i] = b;
+                out[v] = a;
+            }
+        }
+        --i;
+    }
+    if (LDS_ARRAY) {
+        __syncthreads();
+        for (uint32_t k = lane; k < n; k += 64u)
+            out[k] = (int32_t)arr16[k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Union-find state.  A node's code is either its parent's index (non-root)
 // or, for a root, the cluster size and which spanning side (0 none, 1 first,
@@ -357,11 +485,6 @@ __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
             }
         }
     }
-}
-
-__device__ __forceinline__ uint32_t uniform(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // Puts a uniform value into one lane (a compare and a select, no branch).
@@ -1874,5 +1997,29 @@ extern "C" int pc_run_moments(const double *d_q, uint32_t num_p,
     run_moments_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
         d_q, num_p, num_runs, cols, d_mean, d_m2);
     PC_LAUNCH_CHECK("run_moments_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_permute_randomstate(const uint32_t *d_seeds, uint32_t num_runs,
+                                      uint32_t num_edges, int32_t *d_perm,
+                                      int64_t perm_stride, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0 || num_edges == 0)
+        return PC_OK;
+    if (!d_seeds || !d_perm || perm_stride < (int64_t)num_edges)
+        return pc_fail(PC_EINVAL, "pc_permute_randomstate: bad buffer");
+    if (num_edges > 0x7fffffffu)
+        return pc_fail(PC_EOVERFLOW, "pc_permute_randomstate: num_edges >= 2^31");
+    hipStream_t s = as_stream(stream);
+    if (num_edges <= MT_LDS_MAX) {
+        const size_t lds = ((size_t)num_edges * sizeof(uint16_t) + 3u) & ~(size_t)3u;
+        permute_randomstate_kernel<true><<<num_runs, 64, lds, s>>>(
+            d_seeds, num_edges, d_perm, perm_stride);
+    } else {
+        permute_randomstate_kernel<false><<<num_runs, 64, 0, s>>>(
+            d_seeds, num_edges, d_perm, perm_stride);
+    }
+    PC_LAUNCH_CHECK("permute_randomstate_kernel");
     return PC_OK;
 }

@@ -169,3 +169,24 @@ def test_pipeline_matches_single_launch(gpu):
         gpu.cuda.synchronize()
         assert np.array_equal(s1.cpu().numpy(), ref[0])
         assert np.array_equal(s2.cpu().numpy(), ref[1])
+
+
+@pytest.mark.parametrize('M', [1, 2, 12, 480, 32512, 70000])
+def test_randomstate_permutations_bit_exact(gpu, M):
+    """Device MT19937 + legacy shuffle == numpy RandomState(seed).permutation
+    (LDS path for M <= 65536, global path above)."""
+    eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
+        [np.arange(M), np.arange(1, M + 1)], axis=1)))
+    seeds = [0, 1, 42, 12345, 4294967295, 2 ** 31 + 7]
+    got = eng.randomstate_permutations(seeds).cpu().numpy()
+    for r, seed in enumerate(seeds):
+        np.testing.assert_array_equal(
+            got[r], np.random.RandomState(seed).permutation(M), err_msg=str(seed))
+
+
+def test_randomstate_kat_3x3(gpu, golden):
+    """The reference's golden permutation (test_percolate.py:141-142)."""
+    kat = golden('kat_3x3.json')
+    eng = BondEngine(_graph.square_lattice(3))
+    got = eng.randomstate_permutations([kat['seed']]).cpu().numpy()[0]
+    assert got.tolist() == kat['permutation']
