@@ -101,6 +101,30 @@ def _traces_to_structured(ga, perms, mx, sp, mom, spanning_cluster):
     return out
 
 
+def _int_seeds(seeds):
+    """The seeds as uint32 values when every one is an integer in
+    [0, 2**32) (RandomState's integer seeding), else None."""
+    out = []
+    for s in seeds:
+        if isinstance(s, (bool, np.bool_)) or not isinstance(s, (int, np.integer)):
+            return None
+        if s < 0 or s > 0xFFFFFFFF:
+            return None
+        out.append(int(s))
+    return out
+
+
+def _permutations(engine, seeds):
+    """Device tensor of RandomState(seed).permutation(M) for every seed:
+    drawn on the GPU for integer seeds, by numpy on the host otherwise."""
+    ints = _int_seeds(seeds)
+    if ints is not None:
+        return engine.randomstate_permutations(ints)
+    perms = np.stack([np.random.RandomState(seed=s).permutation(engine.M)
+                      for s in seeds])
+    return engine.upload_permutations(perms)
+
+
 def _sweep_host_perms(ga, perms, spanning_cluster):
     """Run the GPU sweep for (R, M) permutations; structured (R, S) array."""
     from ._engine import BondEngine
@@ -182,15 +206,21 @@ def bond_microcanonical_statistics_batch(
     """``bond_microcanonical_statistics`` for many seeds in one GPU sweep:
     structured array (len(seeds), num_edges + 1)."""
     _check_sides(spanning_cluster, spanning_sides)
-    perms = np.stack([np.random.RandomState(seed=s).permutation(num_edges)
-                      for s in seeds]) if len(seeds) else \
-        np.zeros((0, num_edges), dtype=np.int64)
+    seeds = list(seeds)
     ga = _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
                        auxiliary_edge_attributes, spanning_sides)
     if ga.num_nodes != num_nodes or ga.num_edges != num_edges:
         raise ValueError('num_nodes / num_edges do not match perc_graph')
-    return _sweep_host_perms(ga, perms.reshape(len(seeds), num_edges),
-                             spanning_cluster)
+    if not seeds or num_edges == 0:
+        perms = np.zeros((len(seeds), num_edges), dtype=np.int64)
+        return _sweep_host_perms(ga, perms, spanning_cluster)
+    from ._engine import BondEngine
+    eng = BondEngine(ga)
+    d_perm = _permutations(eng, seeds)
+    res = eng.sweep(d_perm)
+    mx, sp, mom = eng.host_traces(res)
+    return _traces_to_structured(ga, d_perm.cpu().numpy(), mx, sp, mom,
+                                 spanning_cluster)
 
 
 def _trace_columns(microcanonical_statistics):
@@ -367,7 +397,7 @@ def bond_canonical_averages_batch(
     percolate/hpc.py:443-702).
 
     With ``seeds`` the runs use numpy RandomState(seed).permutation(M)
-    (bit-exact traces); otherwise ``num_runs`` runs draw device permutations
+    (bit-exact traces; drawn on the GPU for integer seeds); otherwise ``num_runs`` runs draw device permutations
     keyed by ``philox_seed``.  Per chunk of runs: sweep, per-run observables
     over the union of the convolution windows, one FP64 matrix-core
     contraction of all runs' columns with the binomial weights, and the
@@ -410,9 +440,7 @@ def bond_canonical_averages_batch(
     for first in range(0, total, chunk):
         rc = min(chunk, total - first)
         if seeds is not None:
-            perms = np.stack([np.random.RandomState(seed=s).permutation(M)
-                              for s in seeds[first:first + rc]])
-            d_perm = eng.upload_permutations(perms)
+            d_perm = _permutations(eng, seeds[first:first + rc])
         else:
             d_perm = eng.device_permutations(philox_seed, rc, first_run=first)
         res = eng.sweep(d_perm)
