@@ -153,6 +153,10 @@ int pc_expand_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
                                  accumulator added into FP64 every 8 MFMAs  */
 #define PC_PREC_BF16X3_MFMA 3 /* FP32 inputs split into three bf16 terms,
                                  six products on v_mfma_f32_16x16x32_bf16  */
+#define PC_PREC_SPLIT_BF16 4  /* pc_canonical_fused only: weights and columns
+                                 cut into four bytes (exact in bf16), 13
+                                 byte products on v_mfma_f32_16x16x32_bf16
+                                 with exact FP32 sums per 64 steps          */
 
 /*
  * Convolution window of the binomial weights B(n, k; p): k in
@@ -214,6 +218,29 @@ int pc_window_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
  */
 int pc_run_moments(const double *d_q, uint32_t num_p, uint32_t num_runs,
                    uint32_t cols, double *d_mean, double *d_m2, void *stream);
+
+/* Device workspace bytes pc_canonical_fused needs at a precision. */
+int pc_canonical_fused_workspace_size(uint32_t num_p, uint32_t cols,
+                                      int32_t precision, uint64_t *bytes);
+
+/*
+ * Canonical averages with the binomial weights generated inside the
+ * contraction, the whole of _binomial_pmf + the convolution of
+ * percolate/percolate.py:1067-1109,1196-1221 (and hpc.py:496-512) without a
+ * weight matrix: d_out[i * cols + c] = sum_k B(n, k; p_i) X[k][c] /
+ * sum_k B(n, k; p_i), k over the window (d_start[i], d_len[i]) of
+ * pc_binomial_window, X[k][c] = d_x[k * x_stride + c] for k = 0..num_edges
+ * (n = num_edges).  Weights: Loader's form where a lane's run of
+ * consecutive k enters the window, then the exact ratio recursion.
+ * precision: PC_PREC_FP64_MFMA (FP64 matrix cores) or PC_PREC_SPLIT_BF16
+ * (exact byte split on the bf16 matrix cores).  Deterministic; windows of
+ * nearby p share work, so sorted d_ps run fastest.
+ */
+int pc_canonical_fused(uint32_t num_edges, uint32_t cols, const double *d_x,
+                       int64_t x_stride, uint32_t num_p, const double *d_ps,
+                       const int64_t *d_start, const int64_t *d_len,
+                       double *d_out, int32_t precision, void *d_workspace,
+                       uint64_t workspace_bytes, void *stream);
 
 /* FP64 -> FP32 conversion of n elements (inputs of the FP32 precisions). */
 int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,

@@ -423,6 +423,68 @@ def contract(x, weights, precision=nat.PC_PREC_FP64):
     return ContractPlan(x, weights, precision).run()
 
 
+class FusedPlan(object):
+    """Canonical averages of the columns of x ((n + 1, C) float64 device
+    tensor, row k = state after k bonds) at every p, with the binomial
+    weights generated inside the contraction (pc_canonical_fused):
+    out[i, c] = sum_k B(n, k; p_i) x[k, c] / sum_k B(n, k; p_i) over the
+    window of p_i.  The p-values are processed in sorted order (neighbouring
+    windows share the tile's work) and the rows returned in the caller's
+    order.  precision: 'fp64_mfma' or 'split_bf16'."""
+
+    def __init__(self, x, ps, precision='fp64_mfma'):
+        torch = nat.torch_cuda()
+        self.lib = nat.load_library()
+        self.precision = nat.PRECISIONS.get(precision, precision)
+        if self.precision not in (nat.PC_PREC_FP64_MFMA, nat.PC_PREC_SPLIT_BF16):
+            raise ValueError('fused canonical averages run in fp64_mfma or split_bf16')
+        self.x = x.contiguous()
+        self.S, self.C = int(self.x.shape[0]), int(self.x.shape[1])
+        if self.S < 1:
+            raise ValueError('x needs at least one step')
+        self.n = self.S - 1
+        ps = np.atleast_1d(np.asarray(ps, dtype=np.float64))
+        self.P = ps.size
+        order = np.argsort(ps, kind='stable')
+        self.ps = ps
+        start, length = binomial_windows(self.n, ps[order])
+        dev = self.x.device
+        self.d_ps = torch.from_numpy(ps[order].copy()).to(dev)
+        self.d_start = torch.from_numpy(start).to(dev)
+        self.d_len = torch.from_numpy(length).to(dev)
+        self.unsort = torch.from_numpy(np.argsort(order).astype(np.int64)).to(dev)
+        b = ctypes.c_uint64(0)
+        nat.check(self.lib.pc_canonical_fused_workspace_size(
+            self.P, self.C, self.precision, ctypes.byref(b)),
+            'pc_canonical_fused_workspace_size')
+        self.ws_bytes = int(b.value)
+        self.ws = (torch.empty((self.ws_bytes,), dtype=torch.uint8, device=dev)
+                   if self.ws_bytes else None)
+        self.sorted_out = torch.empty((max(self.P, 1), self.C), dtype=torch.float64,
+                                      device=dev)
+        self.out = None
+
+    def launch(self):
+        """Enqueue the kernels (sorted-order result in self.sorted_out)."""
+        torch = nat.torch_cuda()
+        if self.P and self.C:
+            nat.check(self.lib.pc_canonical_fused(
+                self.n, self.C, nat.ptr(self.x), self.C, self.P,
+                nat.ptr(self.d_ps), nat.ptr(self.d_start), nat.ptr(self.d_len),
+                nat.ptr(self.sorted_out), self.precision, nat.ptr(self.ws),
+                self.ws_bytes, nat.stream_handle(torch)), 'pc_canonical_fused')
+
+    def run(self):
+        self.launch()
+        self.out = self.sorted_out[:self.P].index_select(0, self.unsort)
+        return self.out
+
+
+def canonical_fused(x, ps, precision='fp64_mfma'):
+    """See FusedPlan: (P, C) float64 device tensor."""
+    return FusedPlan(x, ps, precision).run()
+
+
 def micro_moments(sum1, sum2, num_edges, num_runs):
     """Mean and sample std (ddof=1) per column and step: (7, S) float64."""
     torch = nat.torch_cuda()

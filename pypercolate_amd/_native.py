@@ -61,6 +61,10 @@ SIGNATURES = {
     'pc_canonical_contract': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32, _vp,
                                              _vp, _vp, _i64, _vp, _i32, _vp,
                                              _u64, _vp]),
+    'pc_canonical_fused_workspace_size': (ctypes.c_int, [_u32, _u32, _i32,
+                                                         ctypes.POINTER(_u64)]),
+    'pc_canonical_fused': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32, _vp, _vp,
+                                          _vp, _vp, _i32, _vp, _u64, _vp]),
     'pc_convert_f64_to_f32': (ctypes.c_int, [_vp, _i64, _vp, _vp]),
     'pc_window_traces': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp,
                                         _i64, _i64, _i32, _vp, _vp]),
@@ -74,8 +78,10 @@ PC_PREC_FP64 = 0
 PC_PREC_FP64_MFMA = 1
 PC_PREC_FP32_MFMA = 2
 PC_PREC_BF16X3_MFMA = 3
+PC_PREC_SPLIT_BF16 = 4
 PRECISIONS = {'fp64': PC_PREC_FP64, 'fp64_mfma': PC_PREC_FP64_MFMA,
-              'fp32_mfma': PC_PREC_FP32_MFMA, 'bf16x3_mfma': PC_PREC_BF16X3_MFMA}
+              'fp32_mfma': PC_PREC_FP32_MFMA, 'bf16x3_mfma': PC_PREC_BF16X3_MFMA,
+              'split_bf16': PC_PREC_SPLIT_BF16}
 
 _lib = None
 

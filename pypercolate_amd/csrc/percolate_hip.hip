@@ -513,7 +513,8 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 //     both of its slots shares no root with any earlier pending candidate:
 //     no earlier merge can change its clusters, so it merges now, in
 //     parallel with the other such candidates (their roots are distinct).
-//     The others re-find their roots and go to the next round.  Hash
+//     The others re-find their roots and go to the next round; the rounds
+//     stop early once one frees at most a single candidate.  Hash
 //     collisions only make the test conservative;
 //  3. the candidates left after `rounds` rounds are resolved in step order
 //     with uniform control flow: roots re-read from the state (one address
@@ -571,6 +572,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 atomicMax(&claims[sv], mine);
             }
             const bool go = cand && claims[su] == mine && claims[sv] == mine;
+            const uint64_t freed = __ballot(go);
             if (go) {
                 uint32_t sa, fa, sb, fb;
                 st.decode(cu, sa, fa);
@@ -599,6 +601,11 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 cand = cand && ru != rv;
             }
             pending = __ballot(cand);
+            // a round that frees at most one candidate is resolving a chain
+            // (typically through the largest cluster's root): the in-order
+            // remainder does that more cheaply
+            if (__builtin_popcountll(freed) <= 1)
+                break;
         }
 
         while (pending) {
@@ -1310,6 +1317,291 @@ __global__ __launch_bounds__(256) void f64_to_f32_kernel(
         out[i] = (float)in[i];
 }
 
+// ---------------------------------------------------------------------------
+// Canonical averages with the binomial weights generated inside the
+// contraction (no weight matrix in HBM):
+//   out[p][c] = sum_k B(n, k; p) X[k][c] / sum_k B(n, k; p)
+// over the window of p.  The normaliser is the contraction of a virtual
+// all-ones column C, so it goes through the same arithmetic as the columns.
+//
+// Tiles: one wavefront = 16 p-values x 16 columns (column C is the ones
+// column) over one of `slices` pieces of the union of the 16 windows.  The
+// four lane groups q = lane / 16 each walk a contiguous quarter of the piece,
+// one k per MFMA k-position, so every lane's weights form a sequence of
+// consecutive k: one direct evaluation (Loader's form, binom_pmf) where the
+// sequence enters the window, then the exact ratio
+//   B(n, k; p) / B(n, k - 1; p) = (n - k + 1) / k * p / (1 - p),
+// with a fresh direct evaluation every FUSED_REANCHOR steps, so the
+// relative error stays below about 1e-13.  The MFMA's k positions only have
+// to agree between the A (weights) and B (columns) operands, which they do
+// because lane (i, q) supplies both for the same k.
+//
+// SPLIT_BF16 (Ozaki-style exact split): per window of 64 k (16 consecutive
+// k in each lane group), a weight is scaled by 2^(32 - E), E per row
+// (max w < 2^E over the row's 64 weights of the window), and a column value
+// by 2^(32 - F), F per column (max |x| < 2^F), rounded to 32-bit integers and
+// cut into four bytes, most significant first.  Bytes are exact in bf16 and
+// their products (< 2^16) exact in FP32; the 13 byte-pair products with
+// s + t <= 4 run on v_mfma_f32_16x16x32_bf16, one FP32 accumulator per level
+// s + t.  Within a window each level sums at most 4 x 64 products of at most
+// 255 x 255, below 2^24, so the FP32 sums are exact integers; they are added
+// into FP64 scaled by 2^(E + F - 16 - 8 (s + t)).  Scales per window keep
+// the far tails of the weights (and small column values) at full relative
+// precision; the dropped levels and the rounding of the 32-bit integers
+// leave a relative error near 1e-10 for non-negative columns.
+// ---------------------------------------------------------------------------
+#define FUSED_REANCHOR 1024
+#define FUSED_F64_UNROLL 8
+
+struct FusedCtx {
+    const double *x;
+    int64_t x_stride;
+    const double *ps;
+    const int64_t *start, *len;
+    uint32_t n, P, C;
+};
+
+// Per-lane walk over consecutive k of one row's weights.
+struct BinomWalk {
+    double n, p, pq, w, anchor;
+    int64_t ka, kend; // in-window k of this lane's range: [ka, kend)
+};
+
+__device__ __forceinline__ void walk_init(BinomWalk &b, const FusedCtx &f,
+                                          uint32_t row, int64_t k0, int64_t k1)
+{
+    b.n = (double)f.n;
+    b.p = 0.5;
+    int64_t st = 0, ln = 0;
+    if (row < f.P) {
+        b.p = f.ps[row];
+        st = f.start[row];
+        ln = f.len[row];
+        // the degenerate distributions hold all their mass at one k
+        if (b.p <= 0.0) {
+            st = 0;
+            ln = 1;
+        } else if (b.p >= 1.0) {
+            st = (int64_t)f.n;
+            ln = 1;
+        }
+    }
+    b.pq = b.p / (1.0 - b.p);
+    b.ka = max(k0, st);
+    b.kend = min(k1, st + ln);
+    b.anchor = b.ka < b.kend ? binom_pmf((double)b.ka, b.n, b.p) : 0.0;
+    b.w = 0.0;
+}
+
+// Weight at k (k advances by one per call); `fresh` replaces the ratio step
+// by a direct evaluation (a uniform branch every FUSED_REANCHOR steps).
+__device__ __forceinline__ double walk_step(BinomWalk &b, int64_t k, bool fresh)
+{
+    const double kd = (double)k;
+    double v;
+    if (fresh && k > b.ka && k < b.kend)
+        v = binom_pmf(kd, b.n, b.p);
+    else
+        v = b.w * ((b.n - kd + 1.0) / kd) * b.pq;
+    v = (k == b.ka) ? b.anchor : v;
+    b.w = v;
+    return (k >= b.ka && k < b.kend) ? v : 0.0;
+}
+
+__device__ __forceinline__ double fused_x(const FusedCtx &f, int64_t k, uint32_t c)
+{
+    if (c < f.C)
+        return f.x[k * f.x_stride + c];
+    return c == f.C ? 1.0 : 0.0;
+}
+
+// K-range of this wave: the union of the tile's windows, cut into `slices`
+// pieces of a multiple of `quant` steps; returns false if the piece is empty.
+__device__ __forceinline__ bool fused_range(const FusedCtx &f, uint32_t p0,
+                                            uint32_t slices, uint32_t slice,
+                                            int64_t quant, int64_t &g_begin,
+                                            int64_t &g_end, int64_t &per)
+{
+    int64_t lo = INT64_MAX, hi = 0;
+    for (uint32_t r = 0; r < 16 && p0 + r < f.P; ++r) {
+        const double p = f.ps[p0 + r];
+        int64_t st = f.start[p0 + r], ln = f.len[p0 + r];
+        if (p <= 0.0) {
+            st = 0;
+            ln = 1;
+        } else if (p >= 1.0) {
+            st = (int64_t)f.n;
+            ln = 1;
+        }
+        lo = min(lo, st);
+        hi = max(hi, st + ln);
+    }
+    if (hi <= lo)
+        return false;
+    per = (hi - lo + slices - 1) / slices;
+    per = (per + quant - 1) / quant * quant;
+    g_begin = lo + (int64_t)slice * per;
+    g_end = min(hi, g_begin + per);
+    return g_begin < g_end;
+}
+
+__global__ __launch_bounds__(64) void canonical_fused_f64_kernel(
+    FusedCtx f, uint32_t slices, double *__restrict__ partial)
+{
+    const uint32_t lane = threadIdx.x, i = lane % 16u, q = lane / 16u;
+    const uint32_t p0 = blockIdx.x * 16u, c0 = blockIdx.y * 16u, slice = blockIdx.z;
+    const uint32_t C1 = f.C + 1u, c = c0 + i;
+    int64_t g_begin = 0, g_end = 0, per = 0;
+    v4f64 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    if (fused_range(f, p0, slices, slice, 4 * FUSED_F64_UNROLL, g_begin, g_end, per)) {
+        const int64_t Lq = per / 4;
+        const int64_t kq0 = g_begin + (int64_t)q * Lq;
+        const int64_t kq1 = min(kq0 + Lq, g_end);
+        BinomWalk b;
+        walk_init(b, f, p0 + i, kq0, kq1);
+        for (int64_t j0 = 0; j0 < Lq; j0 += FUSED_F64_UNROLL) {
+            double a[FUSED_F64_UNROLL], x[FUSED_F64_UNROLL];
+#pragma unroll
+            for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
+                const int64_t k = kq0 + j0 + u;
+                x[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+            }
+            const bool fresh = j0 > 0 && j0 % FUSED_REANCHOR == 0;
+#pragma unroll
+            for (int u = 0; u < FUSED_F64_UNROLL; ++u)
+                a[u] = walk_step(b, kq0 + j0 + u, fresh && u == 0);
+#pragma unroll
+            for (int u = 0; u < FUSED_F64_UNROLL; u += 2) {
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], x[u], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u + 1], x[u + 1], acc1, 0, 0, 0);
+            }
+        }
+    }
+    // v_mfma_f64_16x16x4f64: register r holds D[q + 4r][i]
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t row = p0 + q + 4u * (uint32_t)r;
+        if (row < f.P && c < C1)
+            partial[((int64_t)slice * f.P + row) * C1 + c] = acc0[r] + acc1[r];
+    }
+}
+
+// |v| 2^e rounded to a 32-bit integer (e makes |v| 2^e < 2^32)
+__device__ __forceinline__ uint32_t split_u32(double v, int e)
+{
+    return (uint32_t)fmin(ldexp(fabs(v), e) + 0.5, 4294967295.0);
+}
+
+// byte `s` (0 = most significant) of u as a bf16, negated when neg
+__device__ __forceinline__ __bf16 byte_bf16(uint32_t u, int s, bool neg)
+{
+    const float v = (float)((u >> (24 - 8 * s)) & 255u);
+    return (__bf16)(neg ? -v : v);
+}
+
+__global__ __launch_bounds__(64) void canonical_fused_split_kernel(
+    FusedCtx f, uint32_t slices, double *__restrict__ partial)
+{
+    const uint32_t lane = threadIdx.x, i = lane % 16u, q = lane / 16u;
+    const uint32_t p0 = blockIdx.x * 16u, c0 = blockIdx.y * 16u, slice = blockIdx.z;
+    const uint32_t C1 = f.C + 1u, c = c0 + i;
+    int64_t g_begin = 0, g_end = 0, per = 0;
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    if (fused_range(f, p0, slices, slice, 64, g_begin, g_end, per)) {
+        const int64_t Lq = per / 4;
+        const int64_t kq0 = g_begin + (int64_t)q * Lq;
+        const int64_t kq1 = min(kq0 + Lq, g_end);
+        BinomWalk b;
+        walk_init(b, f, p0 + i, kq0, kq1);
+        for (int64_t j0 = 0; j0 < Lq; j0 += 16) {
+            double x[16], w[16];
+            double m = 0.0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t k = kq0 + j0 + e;
+                x[e] = k < kq1 ? fused_x(f, k, c) : 0.0;
+                m = fmax(m, fabs(x[e]));
+            }
+            const bool fresh = j0 > 0 && j0 % FUSED_REANCHOR == 0;
+            double wm = 0.0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                w[e] = walk_step(b, kq0 + j0 + e, fresh && e == 0);
+                wm = fmax(wm, w[e]);
+            }
+            // scales of column i and row i over the window: lanes i, i+16,
+            // i+32, i+48 hold its four runs of 16 k
+            m = fmax(m, __shfl_xor(m, 16, 64));
+            m = fmax(m, __shfl_xor(m, 32, 64));
+            wm = fmax(wm, __shfl_xor(wm, 16, 64));
+            wm = fmax(wm, __shfl_xor(wm, 32, 64));
+            const int F = m > 0.0 ? ilogb(m) + 1 : 0;
+            const int Ew = wm > 0.0 ? ilogb(wm) + 1 : 0;
+            // output register r holds row 4q + r: its scale from lane 4q + r
+            int Er[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Er[r] = __shfl(Ew, (int)(4u * q) + r, 64);
+            v4f32 acc[5];
+#pragma unroll
+            for (int l = 0; l < 5; ++l)
+                acc[l] = v4f32{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                v8bf16 A[4], B[4];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint32_t uw = split_u32(w[8 * h + e], 32 - Ew);
+                    const uint32_t ux = split_u32(x[8 * h + e], 32 - F);
+                    const bool neg = x[8 * h + e] < 0.0;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        A[s][e] = byte_bf16(uw, s, false);
+                        B[s][e] = byte_bf16(ux, s, neg);
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (s + t <= 4)
+                            acc[s + t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                A[s], B[t], acc[s + t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int l = 0; l < 5; ++l)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    out[r] += ldexp((double)acc[l][r], Er[r] + F - 16 - 8 * l);
+        }
+    }
+    // FP32-accumulating MFMA: register r holds D[4q + r][i]
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t row = p0 + 4u * q + (uint32_t)r;
+        if (row < f.P && c < C1)
+            partial[((int64_t)slice * f.P + row) * C1 + c] = out[r];
+    }
+}
+
+// out[p][c] = sum over slices of partial[.][p][c] / the same sum of the
+// ones column, in a fixed order
+__global__ __launch_bounds__(256) void fused_reduce_kernel(
+    const double *__restrict__ partial, uint32_t slices, uint32_t P, uint32_t C,
+    double *__restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)P * C)
+        return;
+    const int64_t p = idx / C, c = idx % C;
+    const uint32_t C1 = C + 1u;
+    double num = 0.0, den = 0.0;
+    for (uint32_t s = 0; s < slices; ++s) {
+        const double *row = partial + ((int64_t)s * P + p) * C1;
+        num += row[c];
+        den += row[C];
+    }
+    out[idx] = num / den;
+}
+
 
 // ---------------------------------------------------------------------------
 // Per-run observables as FP64 columns over a window of steps, the input of
@@ -1888,6 +2180,60 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
     contract_reduce_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(
         partial, slices, n, d_out);
     PC_LAUNCH_CHECK("contract_reduce_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_canonical_fused_workspace_size(uint32_t num_p, uint32_t cols,
+                                                 int32_t precision, uint64_t *bytes)
+{
+    g_last_error[0] = 0;
+    if (!bytes)
+        return pc_fail(PC_EINVAL, "pc_canonical_fused_workspace_size: bytes is NULL");
+    if (precision != PC_PREC_FP64_MFMA && precision != PC_PREC_SPLIT_BF16)
+        return pc_fail(PC_EINVAL, "pc_canonical_fused_workspace_size: unknown precision %d",
+                       precision);
+    *bytes = (uint64_t)contract_slices(num_p, cols + 1u) * num_p * (cols + 1u) *
+             sizeof(double);
+    return PC_OK;
+}
+
+extern "C" int pc_canonical_fused(uint32_t num_edges, uint32_t cols,
+                                  const double *d_x, int64_t x_stride,
+                                  uint32_t num_p, const double *d_ps,
+                                  const int64_t *d_start, const int64_t *d_len,
+                                  double *d_out, int32_t precision,
+                                  void *d_workspace, uint64_t workspace_bytes,
+                                  void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0 || cols == 0)
+        return PC_OK;
+    if (cols > 4096u)
+        return pc_fail(PC_EOVERFLOW, "pc_canonical_fused: at most 4096 columns");
+    if (!d_x || !d_ps || !d_start || !d_len || !d_out || x_stride < (int64_t)cols)
+        return pc_fail(PC_EINVAL, "pc_canonical_fused: bad buffer");
+    uint64_t need = 0;
+    if (pc_canonical_fused_workspace_size(num_p, cols, precision, &need) != PC_OK)
+        return PC_EINVAL;
+    if (!d_workspace || workspace_bytes < need)
+        return pc_fail(PC_ENOMEM, "pc_canonical_fused: workspace needs %llu bytes",
+                       (unsigned long long)need);
+    hipStream_t s = as_stream(stream);
+    const FusedCtx f{d_x, x_stride, d_ps, d_start, d_len, num_edges, num_p, cols};
+    const uint32_t slices = contract_slices(num_p, cols + 1u);
+    const dim3 grid((num_p + 15u) / 16u, (cols + 1u + 15u) / 16u, slices);
+    double *partial = (double *)d_workspace;
+    if (precision == PC_PREC_FP64_MFMA) {
+        canonical_fused_f64_kernel<<<grid, 64, 0, s>>>(f, slices, partial);
+        PC_LAUNCH_CHECK("canonical_fused_f64_kernel");
+    } else {
+        canonical_fused_split_kernel<<<grid, 64, 0, s>>>(f, slices, partial);
+        PC_LAUNCH_CHECK("canonical_fused_split_kernel");
+    }
+    const int64_t n = (int64_t)num_p * cols;
+    fused_reduce_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(
+        partial, slices, num_p, cols, d_out);
+    PC_LAUNCH_CHECK("fused_reduce_kernel");
     return PC_OK;
 }
 

@@ -1,5 +1,6 @@
-"""Graph builders shared by the tests (networkx 3 API)."""
+"""Graph builders and reference weights shared by the tests."""
 import networkx as nx
+import numpy as np
 
 
 def spanning_2d_grid(length):
@@ -34,3 +35,20 @@ def grid_3x3_graph(spanning, kat):
         g.add_edges_from(kat['aux_side0_edges'], span=0)
         g.add_edges_from(kat['aux_side1_edges'], span=1)
     return g
+
+
+def ratio_weights(n, p, k):
+    """Normalised binomial weights over the consecutive steps k (an integer
+    array) from the exact ratio B(k) / B(k - 1) = (n - k + 1) / k * p / (1 - p),
+    multiplied up in 80-bit long double: relative error below len(k) * 1e-19,
+    independent of any pmf routine (scipy.stats.binom.pmf is off by ~1e-12
+    at n ~ 3e4).  p = 0 and p = 1 put all mass on k = 0 and k = n."""
+    k = np.asarray(k)
+    if p <= 0.0 or p >= 1.0:
+        w = (k == (0 if p <= 0.0 else n)).astype(np.longdouble)
+        return w / w.sum()
+    ld = np.longdouble
+    kk = k[1:].astype(ld)
+    r = (ld(n) - kk + 1) / kk * (ld(p) / (1 - ld(p)))
+    w = np.concatenate([np.ones(1, dtype=ld), np.cumprod(r)])
+    return w / w.sum()

@@ -7,7 +7,7 @@ import pytest
 import scipy.stats
 
 from oracle import oracle
-from tests.helpers import spanning_2d_grid
+from tests.helpers import ratio_weights, spanning_2d_grid
 
 
 @pytest.mark.parametrize('n,p', [(100, 0.1), (480, 0.5), (32512, 0.45), (1000, 0.01)])
@@ -18,6 +18,32 @@ def test_binomial_pmf_vs_scipy(golden, n, p):
     assert got.sum() == pytest.approx(1.0, abs=1e-12)
     big = ref > 1e-250
     np.testing.assert_allclose(got[big], ref[big], rtol=1e-7)
+
+
+@pytest.mark.parametrize('n,p,k0,k1', [(32512, 0.47, 15000, 15600),
+                                        (480, 0.5, 0, 481), (480, 0.3, 100, 190),
+                                        (2095104, 0.3, 628400, 628700),
+                                        (7, 0.0, 0, 8), (7, 1.0, 0, 8)])
+def test_ratio_weights_exact(n, p, k0, k1):
+    """The long-double reference weights of the fused-contraction tests
+    against exact rational arithmetic (Fraction of the binary p), to 1e-15
+    of the largest weight: C(n, k) p^k (1-p)^(n-k) itself for n < 1000, the
+    exact ratio recursion for the large n."""
+    import math
+    from fractions import Fraction
+    k = np.arange(k0, k1)
+    got = ratio_weights(n, p, k).astype(np.float64)
+    P = Fraction(p)
+    Q = 1 - P
+    if n < 1000:
+        terms = [math.comb(n, int(j)) * P ** int(j) * Q ** (n - int(j)) for j in k]
+    else:
+        terms = [Fraction(1)]
+        for j in range(k0 + 1, k1):
+            terms.append(terms[-1] * Fraction(n - j + 1, j) * P / Q)
+    tot = sum(terms)
+    exact = np.array([float(t / tot) for t in terms])
+    assert np.abs(got - exact).max() <= 1e-15 * exact.max()
 
 
 def test_binomial_pmf_edges():
