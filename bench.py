@@ -187,27 +187,46 @@ def canonical_comparison(length, runs, num_p, seed=7, reps=5):
 
 def batch_bench(args):
     """Finite-size batch: L in {16, 32, 64, 128, 256, 512}, 2^26 / L^2 runs
-    each (Philox seed 42), permute -> sweep -> reduce -> convolve (101 p in
-    [0.4, 0.6]) for every L, recorded once as a HIP graph and replayed."""
+    each (Philox seed 42), permute -> sweep -> reduce -> canonical averages
+    (101 p in [0.4, 0.6]) for every L, with autotuned sweep knobs, one
+    stream per L, recorded once as a HIP graph and replayed.  Beside it, the
+    plain per-L GPU form: the same pipelines launched one L after the other
+    on one stream, default knobs, no graph; its exact sums must be
+    bit-identical to the batch's."""
     import torch
     from pypercolate_amd.batch import FiniteSizeBatch, default_runs
     lengths = [16, 32, 64, 128, 256, 512]
     runs = default_runs(lengths)
-    batch = FiniteSizeBatch(lengths, runs, seed=42, ps=np.linspace(0.4, 0.6, 101))
-    batch.capture()
-    for _ in range(args.warmup):
-        batch.run()
-    torch.cuda.synchronize()
+    ps = np.linspace(0.4, 0.6, 101)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.steps):
-        batch.run()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+
+    def timed(run):
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    plain = FiniteSizeBatch(lengths, runs, seed=42, ps=ps, tune=False)
+    plain_ms = timed(plain.run_sequential)
+    plain_res = plain.results()
+    del plain
+    torch.cuda.empty_cache()
+    batch = FiniteSizeBatch(lengths, runs, seed=42, ps=ps)
+    batch.capture()
+    ms = timed(batch.run)
     res = batch.results()
     ok = all(int(r['sum1'][0, -1, 0]) == r['runs'] * L * L for L, r in res.items())
+    exact = all(np.array_equal(res[L]['sum1'], plain_res[L]['sum1']) and
+                np.array_equal(res[L]['sum2'], plain_res[L]['sum2'])
+                for L in lengths)
+    canon = max(float(np.abs(res[L]['canonical'] - plain_res[L]['canonical']).max())
+                for L in lengths)
     return {
         'metric': METRIC + ' [finite-size batch]',
         'value': batch.additions / (ms / 1e3),
@@ -220,6 +239,13 @@ def batch_bench(args):
                                 'one HIP graph per step').format(lengths, runs),
                    'lengths': lengths, 'runs': runs,
                    'parallelism': 'single GPU, one stream per L'},
+        'plain_per_L': {'ms_per_step': plain_ms,
+                        'value': batch.additions / (plain_ms / 1e3),
+                        'form': 'per-L launches in sequence on one stream, '
+                                'default sweep knobs, no HIP graph',
+                        'speedup_of_batch': plain_ms / ms,
+                        'sums_bit_exact_vs_batch': exact,
+                        'canonical_max_abs_diff': canon},
         'tuning': {L: r['tuning'] for L, r in res.items()},
         'sums_check': ok,
     }
@@ -264,12 +290,60 @@ def cpu_baseline(length, sample_runs, seed):
     }
 
 
+def _pysweep_runs(job):
+    from oracle import pysweep
+    length, seeds = job
+    for s in seeds:
+        pysweep.lattice_run(length, s)
+    return len(seeds)
+
+
+def reference_class_baseline(length, single_runs, runs_per_proc):
+    """The reference-class CPU path: a pure-Python restatement of the
+    per-edge loop (oracle/pysweep.py: dict union-find, numpy uint64 moment
+    vectors, per-step arrays), timed single-process on one core and with one
+    process per core (the reference's jug-style parallelism).  Runs before
+    the GPU is touched, so the worker processes are plain forks."""
+    import multiprocessing as mp
+    M = 2 * length * (length - 1)
+    t0 = time.perf_counter()
+    _pysweep_runs((length, list(range(single_runs))))
+    t1 = time.perf_counter() - t0
+    procs = min(os.cpu_count() or 1, 16)
+    jobs = [(length, list(range(100 + i * runs_per_proc,
+                                100 + (i + 1) * runs_per_proc)))
+            for i in range(procs)]
+    t0 = time.perf_counter()
+    with mp.get_context('fork').Pool(procs) as pool:
+        done = sum(pool.map(_pysweep_runs, jobs))
+    tp = time.perf_counter() - t0
+    return {
+        'single_core': {'value': single_runs * M / t1,
+                        'unit': 'bond additions/s', 'cores': 1,
+                        'sample': '{} runs of L={}; {:.2f} s'.format(
+                            single_runs, length, t1)},
+        'all_cores': {'value': done * M / tp, 'unit': 'bond additions/s',
+                      'cores': procs,
+                      'sample': '{} processes x {} runs of L={}; {:.2f} s '
+                                'including process start'.format(
+                                    procs, runs_per_proc, length, tp)},
+        'kind': 'port (pure Python, reference cost class)',
+        'reference_published': {'value': 2.14e4, 'unit': 'bond additions/s',
+                                'cores': 1,
+                                'host': "i7-2620M, the reference's published figure"},
+    }
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU '
-              'path (runs would shard by run; see DESIGN.md)', file=sys.stderr)
+              'path (the runs would shard across GPUs, with one all-reduce of '
+              'the exact sums)', file=sys.stderr)
         return 2
+    ref_class = None
+    if not args.batch and args.cpu_sample_runs > 0:
+        ref_class = reference_class_baseline(args.length, 8, 4)
 
     import torch
     from pypercolate_amd import _graph, _native as nat
@@ -295,16 +369,11 @@ def main(argv=None):
 
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
-    sweep_ms = []
-    launches = 0
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
     start.record()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         pipe.run()
-        # read the previous step's sweep events before they are replaced
-        if k + 1 < args.steps:
-            pass
     stop.record()
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall
@@ -380,6 +449,7 @@ def main(argv=None):
     if args.cpu_sample_runs > 0:
         out['cpu_baseline'] = cpu_baseline(args.length, args.cpu_sample_runs,
                                            args.seed)
+        out['cpu_baseline']['reference_class'] = ref_class
     print(json.dumps(out), flush=True)
     return 0 if ok else 1
 

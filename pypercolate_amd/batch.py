@@ -3,8 +3,9 @@
 ``FiniteSizeBatch(lengths, runs)`` holds, for every L, the square lattice
 spanning_2d_grid(L), its buffers and its tuned sweep knobs, and runs for
 each L the pipeline permute (Philox) -> sweep -> chunk states -> exact
-microcanonical sums -> mean per step -> canonical convolution at the given
-p (FP64 on the matrix cores).  The per-L pipelines are independent: they
+microcanonical sums -> mean per step -> canonical averages at the given p
+(FP64 on the matrix cores, binomial weights generated inside the
+contraction).  The per-L pipelines are independent: they
 are issued on one stream each (default priority), forked from and joined
 back into the caller's stream with events, so the GPU overlaps the long
 large-L sweeps with the many short small-L ones.  ``capture()`` records the
@@ -21,8 +22,7 @@ import numpy as np
 
 from . import _graph
 from . import _native as nat
-from ._engine import (BondEngine, ContractPlan, SweepResult, binomial_weights,
-                      micro_moments)
+from ._engine import BondEngine, FusedPlan, micro_moments
 from .autotune import tune_sweep
 
 
@@ -54,10 +54,9 @@ class _LengthPlan(object):
         self.workspace = (torch.empty((wb,), dtype=torch.uint8, device=dev)
                           if wb else None)
         self.ps = np.asarray(ps, dtype=np.float64)
-        self.weights = binomial_weights(M, self.ps) if self.ps.size else None
         self.x = torch.empty((S, nat.PC_COLUMNS), dtype=torch.float64, device=dev)
-        self.plan = (ContractPlan(self.x, self.weights, nat.PC_PREC_FP64_MFMA)
-                     if self.weights is not None else None)
+        self.plan = (FusedPlan(self.x, self.ps, 'fp64_mfma')
+                     if self.ps.size else None)
         self.mean = None
         self.std = None
 
@@ -123,6 +122,12 @@ class FiniteSizeBatch(object):
             self.graph.replay()
         else:
             self._launch_all()
+
+    def run_sequential(self):
+        """The plain per-L form: every L's pipeline in turn on the current
+        stream (no per-L streams, no graph)."""
+        for plan in self.plans:
+            plan.launch()
 
     @property
     def additions(self):

@@ -462,7 +462,8 @@ def canonical_averages(ps, microcanonical_averages_arrays):
     """Canonical cluster statistics from microcanonical ones
     (percolate/percolate.py:1112-1223): every array (means and confidence
     bounds) convolved with the binomial weights of each p, on the GPU in
-    FP64."""
+    FP64 with the weights generated inside the contraction
+    (pc_canonical_fused)."""
     from . import _engine as E
     import torch
     arrays = microcanonical_averages_arrays
@@ -498,9 +499,8 @@ def canonical_averages(ps, microcanonical_averages_arrays):
     x = np.ascontiguousarray(np.concatenate(cols, axis=1)) if cols else \
         np.zeros((num_edges + 1, 0))
     if ps.size and x.shape[1]:
-        wt = E.binomial_weights(num_edges, ps.reshape(-1))
-        dx = torch.from_numpy(x).to(wt.w.device)
-        q = E.contract(dx, wt).cpu().numpy()
+        dx = torch.from_numpy(x).to(E.nat.torch_cuda().device('cuda'))
+        q = E.canonical_fused(dx, ps.reshape(-1), 'fp64_mfma').cpu().numpy()
     else:
         q = np.zeros((ps.size, x.shape[1]))
     for key, off, width in layout:

@@ -151,3 +151,27 @@ def test_moments_wrap_mod_2_64():
     assert tr['max_cluster_size'][0, s] == big
     assert tr['moments'][0, s].tolist() == expect.tolist()
     assert int(expect[4]) == singles  # 2**64 wrapped to 0
+
+
+@pytest.mark.parametrize('length', [1, 2, 6, 9])
+def test_python_restatement_matches_c_oracle(length):
+    """The reference-class Python sweep (oracle/pysweep.py, timed by
+    bench.py) gives the C oracle's traces on spanning_2d_grid(L)."""
+    from oracle import pysweep
+    from pypercolate_amd import _graph
+    ga = _graph.square_lattice(length)
+    L, N, M = length, ga.num_nodes, ga.num_edges
+    g = {'num_nodes': N, 'num_edges': M, 'spanning': True,
+         'edges': ga.edges.astype(np.int64),
+         'aux_side': np.array([0] * L + [1] * L, dtype=np.int32),
+         'aux_edges': np.array([(i, N + i) for i in range(L)] +
+                               [(N - L + i, N + L + i) for i in range(L)],
+                               dtype=np.int64).reshape(-1, 2),
+         'aux_edge_side': np.array([0] * L + [1] * L, dtype=np.int32)}
+    for seed in range(4):
+        perm = np.random.RandomState(seed).permutation(M)
+        tr = oracle.bond_runs(g, perm[None, :])
+        mx, mom, span = pysweep.lattice_run(length, seed)
+        np.testing.assert_array_equal(mx, tr['max_cluster_size'][0])
+        np.testing.assert_array_equal(mom, tr['moments'][0])
+        np.testing.assert_array_equal(span, tr['has_spanning_cluster'][0])
