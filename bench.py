@@ -49,11 +49,14 @@ def parse_args(argv=None):
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a 1 GiB device copy (for PMC scaling)')
     ap.add_argument('--canonical-length', type=int, default=1024,
-                    help='lattice of the canonical-average comparison (0 skips)')
-    ap.add_argument('--canonical-runs', type=int, default=64)
+                    help='lattice of the large-lattice run whose exact sums '
+                         'feed the canonical-average comparison (0 skips both)')
+    ap.add_argument('--canonical-runs', type=int, default=1000)
     ap.add_argument('--canonical-ps', type=int, default=4096)
-    ap.add_argument('--no-tune', dest='tune', action='store_false',
-                    help='use the default sweep knobs instead of the autotuner')
+    ap.add_argument('--tune', action='store_true',
+                    help='pick the sweep knobs with the autotuner first (the '
+                         'default knobs are within 1%% of the tuned ones at '
+                         'L = 128; the finite-size batch always tunes)')
     ap.add_argument('--batch', action='store_true',
                     help='time the finite-size batch (L = 16..512, 2^26/L^2 '
                          'runs each, one HIP graph) instead of L = 128')
@@ -73,9 +76,57 @@ def profiled_traffic_per_add(length, runs):
     return None
 
 
-def canonical_comparison(length, runs, num_p, seed=7, reps=5):
+def large_lattice(length, runs, seed, steps=2, warmup=1):
+    """The L = 1024 configuration: spanning_2d_grid(1024) (N = 1,048,576,
+    M = 2,095,104), 1000 runs from Philox seed 7, union-find state in the
+    HBM workspace (32-bit codes), 64-bit merge records, the whole per-step
+    trace reduced on the device to exact sums (moments k = 3, 4 and all
+    squares in 128/192-bit).  Returns the timing and the engine and exact
+    sums of the last step (the input of the canonical comparison)."""
+    import torch
+    from pypercolate_amd import _graph
+    from pypercolate_amd._engine import BondEngine, Pipeline
+    eng = BondEngine(_graph.square_lattice(length))
+    pipe = Pipeline(eng, runs, seed=seed)
+    for _ in range(warmup):
+        pipe.run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        pipe.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    sweep = pipe.sweep_ms()
+    s1 = pipe.sum1.cpu().numpy().view('uint64')
+    ok = int(s1[0, eng.M, 0]) == runs * eng.N and int(s1[1, eng.M, 0]) == runs
+    adds = float(runs) * eng.M
+    info = {
+        'workload': ('bond percolation, L={} square lattice (N={}, M={}), {} '
+                     'runs from Philox seed {}: permute -> sweep (HBM state) '
+                     '-> chunk states -> exact microcanonical sums').format(
+                         length, eng.N, eng.M, runs, seed),
+        'value': adds / (ms / 1e3), 'unit': 'bond additions/s',
+        'ms_per_step': ms, 'steps': steps, 'warmup': warmup,
+        'sweep_ms_per_step': sweep, 'sweep_launches': len(pipe.sweep_events),
+        'sweep_adds_per_s': adds / (sweep / 1e3),
+        'roofline_sweep': {'bound': 'hbm', 'alg_bytes_per_add': ALG_BYTES_PER_ADD_HBM,
+                           'achieved': ALG_BYTES_PER_ADD_HBM * adds / (sweep / 1e3) / 1e9,
+                           'peak': HBM_PEAK_GBS, 'unit': 'GB/s'},
+        'sums_check': ok,
+    }
+    info['roofline_sweep']['frac'] = info['roofline_sweep']['achieved'] / HBM_PEAK_GBS
+    sums = (pipe.sum1, pipe.sum2)
+    del pipe
+    torch.cuda.empty_cache()
+    return info, eng, sums, runs
+
+
+def canonical_comparison(eng, sums, runs, num_p, reps=5):
     """The canonical average at every precision on real microcanonical data:
-    L x L lattice, `runs` runs (device permutations), 14 columns (mean and
+    the exact sums of `runs` runs on eng's lattice, 14 columns (mean and
     mean square of the 7 observables per step), num_p values of p in
     [0.3, 0.7].
 
@@ -96,9 +147,7 @@ def canonical_comparison(length, runs, num_p, seed=7, reps=5):
     from pypercolate_amd import _graph, _native as nat
     from pypercolate_amd import _engine as E
 
-    eng = E.BondEngine(_graph.square_lattice(length))
-    res = eng.sweep(eng.device_permutations(seed, runs))
-    s1, s2 = eng.microcanonical_sums(res)
+    s1, s2 = sums
     mean, std = E.micro_moments(s1, s2, eng.M, runs)
     second = std * std * ((runs - 1.0) / runs) + mean * mean
     x = torch.cat([mean, second], dim=0).t().contiguous()   # (S, 14)
@@ -144,7 +193,8 @@ def canonical_comparison(length, runs, num_p, seed=7, reps=5):
     scale = np.maximum(mag, np.where(floor > 0, floor, 1.0))
     out = {'workload': ('L={} (M={}), {} runs, {} p in [0.3, 0.7], 14 columns, '
                         'windows |k - Mp| <= 12 sigma + 48 (mean {:.0f} steps)'
-                        ).format(length, eng.M, runs, num_p, lens.mean()),
+                        ).format(int(round(eng.N ** 0.5)), eng.M, runs, num_p,
+                                 lens.mean()),
            'weights_ms': weights_ms, 'convert_f32_ms': convert_ms,
            'weight_evaluations': float(lens.sum()),
            'algorithmic_flops': flops, 'forms': {}}
@@ -443,9 +493,15 @@ def main(argv=None):
         'sums_check': ok,
         'wall_s': t_wall,
     }
+    del pipe
+    torch.cuda.empty_cache()
     if args.canonical_length > 0:
-        out['canonical'] = canonical_comparison(
-            args.canonical_length, args.canonical_runs, args.canonical_ps)
+        info, big, sums, runs = large_lattice(args.canonical_length,
+                                              args.canonical_runs, seed=7)
+        out['config_L{}'.format(args.canonical_length)] = info
+        out['canonical'] = canonical_comparison(big, sums, runs,
+                                                args.canonical_ps)
+        del sums
     if args.cpu_sample_runs > 0:
         out['cpu_baseline'] = cpu_baseline(args.length, args.cpu_sample_runs,
                                            args.seed)

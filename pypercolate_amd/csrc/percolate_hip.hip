@@ -1361,6 +1361,15 @@ struct FusedCtx {
     uint32_t n, P, C;
 };
 
+// Direct evaluations in the walks are rare (one per lane and range, and one
+// per FUSED_REANCHOR steps): kept out of line so the contraction loops keep
+// their registers (112 instead of 160 VGPRs, 4 waves per SIMD).
+__device__ __attribute__((noinline)) double binom_pmf_outline(double x, double n,
+                                                               double p)
+{
+    return binom_pmf(x, n, p);
+}
+
 // Per-lane walk over consecutive k of one row's weights.
 struct BinomWalk {
     double n, p, pq, w, anchor;
@@ -1389,7 +1398,7 @@ __device__ __forceinline__ void walk_init(BinomWalk &b, const FusedCtx &f,
     b.pq = b.p / (1.0 - b.p);
     b.ka = max(k0, st);
     b.kend = min(k1, st + ln);
-    b.anchor = b.ka < b.kend ? binom_pmf((double)b.ka, b.n, b.p) : 0.0;
+    b.anchor = b.ka < b.kend ? binom_pmf_outline((double)b.ka, b.n, b.p) : 0.0;
     b.w = 0.0;
 }
 
@@ -1400,12 +1409,22 @@ __device__ __forceinline__ double walk_step(BinomWalk &b, int64_t k, bool fresh)
     const double kd = (double)k;
     double v;
     if (fresh && k > b.ka && k < b.kend)
-        v = binom_pmf(kd, b.n, b.p);
+        v = binom_pmf_outline(kd, b.n, b.p);
     else
         v = b.w * ((b.n - kd + 1.0) / kd) * b.pq;
     v = (k == b.ka) ? b.anchor : v;
     b.w = v;
     return (k >= b.ka && k < b.kend) ? v : 0.0;
+}
+
+// Row tile of workgroup b out of T: workgroups are dealt to the 8 XCDs in
+// turn, so give each XCD a contiguous run of tiles; neighbouring p-values
+// read the same rows of X and then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
+{
+    const uint32_t x = b % 8u, idx = b / 8u;
+    const uint32_t base = T / 8u, extra = T % 8u;
+    return x * base + min(x, extra) + idx;
 }
 
 __device__ __forceinline__ double fused_x(const FusedCtx &f, int64_t k, uint32_t c)
@@ -1449,7 +1468,8 @@ __global__ __launch_bounds__(64) void canonical_fused_f64_kernel(
     FusedCtx f, uint32_t slices, double *__restrict__ partial)
 {
     const uint32_t lane = threadIdx.x, i = lane % 16u, q = lane / 16u;
-    const uint32_t p0 = blockIdx.x * 16u, c0 = blockIdx.y * 16u, slice = blockIdx.z;
+    const uint32_t p0 = xcd_tile(blockIdx.x, gridDim.x) * 16u;
+    const uint32_t c0 = blockIdx.y * 16u, slice = blockIdx.z;
     const uint32_t C1 = f.C + 1u, c = c0 + i;
     int64_t g_begin = 0, g_end = 0, per = 0;
     v4f64 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
@@ -1502,7 +1522,8 @@ __global__ __launch_bounds__(64) void canonical_fused_split_kernel(
     FusedCtx f, uint32_t slices, double *__restrict__ partial)
 {
     const uint32_t lane = threadIdx.x, i = lane % 16u, q = lane / 16u;
-    const uint32_t p0 = blockIdx.x * 16u, c0 = blockIdx.y * 16u, slice = blockIdx.z;
+    const uint32_t p0 = xcd_tile(blockIdx.x, gridDim.x) * 16u;
+    const uint32_t c0 = blockIdx.y * 16u, slice = blockIdx.z;
     const uint32_t C1 = f.C + 1u, c = c0 + i;
     int64_t g_begin = 0, g_end = 0, per = 0;
     double out[4] = {0.0, 0.0, 0.0, 0.0};
