@@ -495,6 +495,26 @@ def main(argv=None):
     }
     del pipe
     torch.cuda.empty_cache()
+    # bit-exact mode: numpy RandomState(r).permutation(M) for run r, drawn on
+    # the device (MT19937 + the legacy shuffle), same pipeline otherwise
+    exact = Pipeline(eng, R, seeds=np.arange(R))
+    exact.run()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        exact.run()
+    stop.record()
+    torch.cuda.synchronize()
+    ex_ms = start.elapsed_time(stop) / args.steps
+    ex1 = exact.sum1.cpu().numpy().view('uint64')
+    out['bitexact_mode'] = {
+        'value': adds / (ex_ms / 1e3), 'unit': 'bond additions/s',
+        'ms_per_step': ex_ms,
+        'permutations': 'numpy RandomState(r).permutation(M), r = 0..{}, '
+                        'on the device (MT19937 + legacy shuffle)'.format(R - 1),
+        'sums_check': bool(int(ex1[0, M, 0]) == R * N and int(ex1[1, M, 0]) == R)}
+    del exact
+    torch.cuda.empty_cache()
     if args.canonical_length > 0:
         info, big, sums, runs = large_lattice(args.canonical_length,
                                               args.canonical_runs, seed=7)

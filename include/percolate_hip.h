@@ -94,14 +94,17 @@ int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
  * (pc_sweep_workspace_size bytes) per state_mode (PC_STATE_*).
  * merge_rounds is the number of parallel conflict-free merge rounds per
  * batch of 64 bonds before the in-order remainder (0 = library default);
- * results do not depend on it or on state_mode.
+ * results do not depend on it or on state_mode.  With d_chunk_state
+ * non-NULL the sweep also writes the chunk states of pc_chunk_scan (same
+ * layout, d_chunk_state[(r * C + c) * 8 + w]), so that pass is not needed.
  */
 int pc_sweep(uint32_t num_nodes, uint32_t num_edges, const int32_t *d_edges,
              const uint8_t *d_node_sides, int32_t span_on_first_merge,
              const int32_t *d_perm, int64_t perm_stride, uint32_t num_runs,
              void *d_records, int64_t rec_stride, uint32_t *d_span_step,
              void *d_workspace, uint64_t workspace_bytes,
-             uint32_t merge_rounds, uint32_t state_mode, void *stream);
+             uint32_t merge_rounds, uint32_t state_mode,
+             uint64_t *d_chunk_state, void *stream);
 
 /* Resident sweep workgroups (runs) per compute unit for a graph size. */
 int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,

@@ -83,20 +83,27 @@ class BondEngine(object):
             self.M, nat.ptr(out), self.M, st), 'pc_permute_philox')
         return out
 
-    def randomstate_permutations(self, seeds, out=None):
-        """numpy RandomState(seed).permutation(M) for every seed, drawn on
-        the device (bit-identical to numpy's legacy generator)."""
-        torch = self.torch
+    def upload_seeds(self, seeds):
+        """RandomState seeds (integers in [0, 2**32)) as a device uint32
+        vector (an int32 tensor holding the bits)."""
         seeds = np.asarray(seeds, dtype=np.int64).reshape(-1)
         if seeds.size and (seeds.min() < 0 or seeds.max() > 0xFFFFFFFF):
             raise ValueError('RandomState seeds must be in [0, 2**32 - 1]')
-        d_seeds = torch.from_numpy(seeds.astype(np.uint32).view(np.int32)).to(
+        return self.torch.from_numpy(seeds.astype(np.uint32).view(np.int32)).to(
             self.device)
+
+    def randomstate_permutations(self, seeds, out=None):
+        """numpy RandomState(seed).permutation(M) for every seed, drawn on
+        the device (bit-identical to numpy's legacy generator).  seeds: host
+        integers, or a device vector from upload_seeds."""
+        torch = self.torch
+        d_seeds = seeds if torch.is_tensor(seeds) else self.upload_seeds(seeds)
+        n = int(d_seeds.numel())
         if out is None:
-            out = torch.empty((seeds.size, max(self.M, 1)), dtype=torch.int32,
+            out = torch.empty((n, max(self.M, 1)), dtype=torch.int32,
                               device=self.device)
         nat.check(self.lib.pc_permute_randomstate(
-            nat.ptr(d_seeds), seeds.size, self.M, nat.ptr(out), max(self.M, 1),
+            nat.ptr(d_seeds), n, self.M, nat.ptr(out), max(self.M, 1),
             nat.stream_handle(torch)), 'pc_permute_randomstate')
         return out
 
@@ -117,9 +124,26 @@ class BondEngine(object):
                   'pc_sweep_occupancy')
         return int(v.value)
 
-    def sweep(self, d_perm, records=None, span_step=None, workspace=None):
+    def fuse_chunk_states(self):
+        """Whether the sweep should write the chunk states itself.  Measured
+        on MI355X: with the union-find state in LDS (one run per SIMD, every
+        cycle on the critical path) the extra wave reductions slow the sweep
+        by ~25 %, more than the separate pass costs when it runs on another
+        stream; with the state in HBM the sweep hides them and the separate
+        pass (one wavefront walking each long run) is the slower choice."""
+        lds = self.N <= 16384 and self.state_mode != nat.PC_STATE_HBM
+        return not lds
+
+    def sweep(self, d_perm, records=None, span_step=None, workspace=None,
+              chunk_state=None, chunks=None):
+        """Sweep the runs of d_perm (R, M).  With chunks (default: whether
+        fuse_chunk_states()) the sweep also writes the chunk states (into
+        chunk_state, or a new tensor), so the result needs no chunk_scan
+        pass; otherwise they are computed on first use."""
         torch = self.torch
         R = int(d_perm.shape[0])
+        if chunks is None:
+            chunks = chunk_state is not None or self.fuse_chunk_states()
         if records is None:
             records = torch.empty((R, max(self.M, 1)), dtype=self.rec_torch,
                                   device=self.device)
@@ -130,14 +154,22 @@ class BondEngine(object):
             workspace = torch.empty((wb,), dtype=torch.uint8, device=self.device)
         elif workspace is not None:
             wb = int(workspace.numel())
+        if chunks and chunk_state is None:
+            C = max((self.M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
+            chunk_state = torch.empty((R, C, nat.PC_STATE_WORDS),
+                                      dtype=torch.int64, device=self.device)
         st = nat.stream_handle(torch)
         nat.check(self.lib.pc_sweep(
             self.N, self.M, nat.ptr(self.d_edges), nat.ptr(self.d_sides),
             self.span_first, nat.ptr(d_perm), max(self.M, 1), R,
             nat.ptr(records), max(self.M, 1), nat.ptr(span_step),
-            nat.ptr(workspace), wb, self.merge_rounds, self.state_mode, st),
+            nat.ptr(workspace), wb, self.merge_rounds, self.state_mode,
+            nat.ptr(chunk_state) if chunks else None, st),
             'pc_sweep')
-        return SweepResult(self, R, records, span_step, d_perm)
+        res = SweepResult(self, R, records, span_step, d_perm)
+        if chunks:
+            res.chunk_state = chunk_state
+        return res
 
     def chunk_scan(self, res, out=None):
         torch = self.torch
@@ -208,20 +240,30 @@ def host_sums(sum1, sum2):
 
 
 class Pipeline(object):
-    """R runs of one graph through permute -> sweep -> chunk states ->
-    exact sums, in sub-batches on three streams so that the reduction of one
+    """R runs of one graph through permute -> sweep -> chunk states -> exact
+    sums, in sub-batches on three streams so that the reduction of one
     sub-batch runs beside the sweep of the next (the sweep leaves most issue
     slots of a CU idle: its occupancy is bounded by LDS, four runs per CU at
     N = 16384).  Stream order: permutations on one stream, sweeps on a
     second (each waiting for its permutations), chunk states and sums on a
     third (each waiting for its sweep, accumulating into one pair of sum
-    buffers).  All buffers are allocated up front."""
+    buffers).  HBM-state graphs get their chunk states from the sweep itself
+    (BondEngine.fuse_chunk_states).  All buffers are allocated up front.
 
-    def __init__(self, engine, num_runs, seed=0, sub_batch=None):
+    Permutations: Philox-keyed (pc_permute_philox, seed) by default, or
+    with seeds= (R integers) numpy RandomState(seeds[r]).permutation(M),
+    bit-identical to the reference's draw, on the device."""
+
+    def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None):
         torch = self.torch = engine.torch
         self.engine = eng = engine
         self.R = int(num_runs)
         self.seed = seed
+        self.d_seeds = None
+        if seeds is not None:
+            self.d_seeds = eng.upload_seeds(seeds)
+            if int(self.d_seeds.numel()) != self.R:
+                raise ValueError('need one seed per run')
         dev = eng.device
         M, S = eng.M, eng.S
         if sub_batch is None:
@@ -257,11 +299,15 @@ class Pipeline(object):
             s.wait_event(start)
         self.sweep_events = []
         first = True
+        fused = eng.fuse_chunk_states()
         for b0 in range(0, self.R, self.sub):
             n = min(self.sub, self.R - b0)
             perm = self.perm[b0:b0 + n]
             with torch.cuda.stream(sp):
-                eng.device_permutations(self.seed, n, first_run=b0, out=perm)
+                if self.d_seeds is None:
+                    eng.device_permutations(self.seed, n, first_run=b0, out=perm)
+                else:
+                    eng.randomstate_permutations(self.d_seeds[b0:b0 + n], out=perm)
                 have_perm = torch.cuda.Event()
                 have_perm.record(sp)
             ss.wait_event(have_perm)
@@ -270,13 +316,16 @@ class Pipeline(object):
                 e0.record(ss)
                 res = eng.sweep(perm, records=self.records[b0:b0 + n],
                                 span_step=self.span[b0:b0 + n],
-                                workspace=self.workspace)
+                                workspace=self.workspace,
+                                chunk_state=self.state[b0:b0 + n] if fused else None,
+                                chunks=fused)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e1.record(ss)
             self.sweep_events.append((e0, e1))
             sr.wait_event(e1)
             with torch.cuda.stream(sr):
-                res.chunk_state = eng.chunk_scan(res, out=self.state[b0:b0 + n])
+                if not fused:
+                    res.chunk_state = eng.chunk_scan(res, out=self.state[b0:b0 + n])
                 eng.microcanonical_sums(res, sum1=self.sum1, sum2=self.sum2,
                                         accumulate=not first)
             first = False

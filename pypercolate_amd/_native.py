@@ -44,7 +44,8 @@ SIGNATURES = {
     'pc_sweep_workspace_size': (ctypes.c_int, [_u32, _u32, _u32,
                                                ctypes.POINTER(_u64)]),
     'pc_sweep': (ctypes.c_int, [_u32, _u32, _vp, _vp, _i32, _vp, _i64, _u32,
-                                _vp, _i64, _vp, _vp, _u64, _u32, _u32, _vp]),
+                                _vp, _i64, _vp, _vp, _u64, _u32, _u32, _vp,
+                                _vp]),
     'pc_sweep_occupancy': (ctypes.c_int, [_u32, _u32, ctypes.POINTER(_i32)]),
     'pc_chunk_scan': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp]),
     'pc_microcanonical_sums': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp,

@@ -64,9 +64,12 @@ class _LengthPlan(object):
         """Enqueue this L's pipeline on the current stream."""
         eng = self.engine
         eng.device_permutations(self.seed, self.runs, out=self.perm)
+        fused = eng.fuse_chunk_states()
         res = eng.sweep(self.perm, records=self.records, span_step=self.span,
-                        workspace=self.workspace)
-        res.chunk_state = eng.chunk_scan(res, out=self.state)
+                        workspace=self.workspace,
+                        chunk_state=self.state if fused else None, chunks=fused)
+        if not fused:
+            res.chunk_state = eng.chunk_scan(res, out=self.state)
         eng.microcanonical_sums(res, sum1=self.sum1, sum2=self.sum2)
         self.mean, self.std = micro_moments(self.sum1, self.sum2, eng.M,
                                             self.runs)

@@ -487,6 +487,57 @@ __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Merge-record arithmetic.  WIDE (N > 16384) keeps S3, S4 and the moments in
+// 128 bits; otherwise every Sk fits 64 bits (Sk <= N**k <= 2**56).
+// ---------------------------------------------------------------------------
+template <bool WIDE>
+struct Wide {
+    using T = typename std::conditional<WIDE, u128, uint64_t>::type;
+};
+
+template <bool WIDE>
+struct Deltas {
+    using MT = typename Wide<WIDE>::T;
+    uint32_t dmax; // size of the merged cluster (0: no merge)
+    uint32_t cnt;  // 1 for a merge
+    uint64_t d2;   // (a+b)^2 - a^2 - b^2
+    MT d3, d4;
+    template <typename RecT>
+    __device__ __forceinline__ explicit Deltas(RecT rec)
+    {
+        uint32_t a, b;
+        unpack_record(rec, a, b);
+        const bool m = rec != 0;
+        dmax = m ? a + b : 0u;
+        cnt = m ? 1u : 0u;
+        const uint64_t ab = (uint64_t)a * b;
+        const uint64_t aa = (uint64_t)a * a, bb = (uint64_t)b * b;
+        d2 = 2u * ab;
+        d3 = (MT)3u * (MT)ab * (MT)(a + b);
+        d4 = (MT)(2u * ab) * (MT)(2u * aa + 3u * ab + 2u * bb);
+    }
+};
+
+// Word w of a chunk-state entry: {max cluster size, merges, S2, S3 lo,
+// S3 hi, S4 lo, S4 hi, 0}.
+template <typename MT>
+__device__ __forceinline__ uint64_t state_word(uint32_t w, uint64_t mx,
+                                               uint64_t merges, uint64_t s2,
+                                               MT s3, MT s4)
+{
+    switch (w) {
+    case 0: return mx;
+    case 1: return merges;
+    case 2: return s2;
+    case 3: return (uint64_t)s3;
+    case 4: return (uint64_t)((u128)s3 >> 64);
+    case 5: return (uint64_t)s4;
+    case 6: return (uint64_t)((u128)s4 >> 64);
+    default: return 0u;
+    }
+}
+
 // Puts a uniform value into one lane (a compare and a select, no branch).
 template <typename T>
 __device__ __forceinline__ T write_lane(T v, uint32_t lane, T old)
@@ -523,7 +574,11 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 //     registers, lane 0 writing it back, the record placed with a lane
 //     select.
 // Per-step output is one merge record; the spanning step is kept scalar.
-template <class St, typename RecT>
+// With chunk_out, the run's state at the start of every batch of 64 bonds
+// (the chunk state of chunk_scan) is written as well: the batch's records
+// are in registers at its end, so their deltas are reduced across the wave
+// there and no pass over the records is needed.
+template <class St, typename RecT, bool CHUNKS>
 __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           const int2 *__restrict__ edges,
                                           const uint8_t *__restrict__ sides,
@@ -531,9 +586,15 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           uint32_t *__restrict__ claims,
                                           const int32_t *__restrict__ perm,
                                           RecT *__restrict__ rec_out,
-                                          uint32_t *__restrict__ span_out)
+                                          uint32_t *__restrict__ span_out,
+                                          uint64_t *__restrict__ chunk_out)
 {
+    // HBM-state graphs (N > 16384) need 128-bit S3 and S4
+    constexpr bool WIDE = std::is_same<St, GlobalState>::value;
+    using MT = typename Wide<WIDE>::T;
     const uint32_t lane = threadIdx.x;
+    uint64_t cs_mx = 1, cs_merges = 0, cs_s2 = N;
+    MT cs_s3 = N, cs_s4 = N;
     for (uint32_t i = lane; i < N; i += 64u)
         st.set(i, st.root(1u, sides ? (uint32_t)sides[i] : 0u));
     for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
@@ -553,6 +614,9 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     for (uint32_t base = 0; base < M; base += 64u) {
         const uint32_t n = base + lane;
         const bool active = n < M;
+        if (CHUNKS && lane < PC_STATE_WORDS)
+            chunk_out[(size_t)(base / 64u) * PC_STATE_WORDS + lane] =
+                state_word(lane, cs_mx, cs_merges, cs_s2, cs_s3, cs_s4);
         const int2 uv = uv_next;
         uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
         e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
@@ -641,79 +705,60 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         }
         if (active)
             rec_out[n] = rec;
+        if (CHUNKS) {
+            const Deltas<WIDE> d(rec);
+            cs_mx = umax64(cs_mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
+            cs_merges += read_lane(wave_scan_add(d.cnt), 63u);
+            cs_s2 += read_lane(wave_scan_add(d.d2), 63u);
+            cs_s3 += read_lane(wave_scan_add(d.d3), 63u);
+            cs_s4 += read_lane(wave_scan_add(d.d4), 63u);
+        }
         __syncthreads();
     }
     if (lane == 0)
         *span_out = span_at;
 }
 
-template <typename RecT>
+template <typename RecT, bool CHUNKS>
 __global__ __launch_bounds__(64) void sweep_lds_kernel(
     uint32_t N, uint32_t M, const int2 *__restrict__ edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
-    uint32_t *__restrict__ span_step, uint32_t rounds)
+    uint32_t *__restrict__ span_step, uint32_t rounds,
+    uint64_t *__restrict__ chunk_state)
 {
     // claim table first (word aligned), then the 16-bit codes
     extern __shared__ uint32_t lds_words[];
     const uint32_t r = blockIdx.x;
+    const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const LdsState st{(uint16_t *)(lds_words + CLAIM_SLOTS), N};
-    sweep_run(st, N, M, edges, sides, span_first != 0, rounds, lds_words,
-              perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
-              span_step + r);
+    sweep_run<LdsState, RecT, CHUNKS>(
+        st, N, M, edges, sides, span_first != 0, rounds, lds_words,
+        perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
+        span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
 }
 
-template <typename RecT>
+template <typename RecT, bool CHUNKS>
 __global__ __launch_bounds__(64) void sweep_global_kernel(
     uint32_t N, uint32_t M, uint32_t R, const int2 *__restrict__ edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
     uint32_t *__restrict__ span_step, uint32_t *__restrict__ workspace,
-    uint32_t rounds)
+    uint32_t rounds, uint64_t *__restrict__ chunk_state)
 {
     __shared__ uint32_t claims[CLAIM_SLOTS];
+    const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
-        sweep_run(st, N, M, edges, sides, span_first != 0, rounds, claims,
-                  perm + (int64_t)r * perm_stride,
-                  records + (int64_t)r * rec_stride, span_step + r);
+        sweep_run<GlobalState, RecT, CHUNKS>(
+            st, N, M, edges, sides, span_first != 0, rounds, claims,
+            perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
+            span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
         __syncthreads();
     }
 }
-
-// ---------------------------------------------------------------------------
-// Merge-record arithmetic.  WIDE (N > 16384) keeps S3, S4 and the moments in
-// 128 bits; otherwise every Sk fits 64 bits (Sk <= N**k <= 2**56).
-// ---------------------------------------------------------------------------
-template <bool WIDE>
-struct Wide {
-    using T = typename std::conditional<WIDE, u128, uint64_t>::type;
-};
-
-template <bool WIDE>
-struct Deltas {
-    using MT = typename Wide<WIDE>::T;
-    uint32_t dmax; // size of the merged cluster (0: no merge)
-    uint32_t cnt;  // 1 for a merge
-    uint64_t d2;   // (a+b)^2 - a^2 - b^2
-    MT d3, d4;
-    template <typename RecT>
-    __device__ __forceinline__ explicit Deltas(RecT rec)
-    {
-        uint32_t a, b;
-        unpack_record(rec, a, b);
-        const bool m = rec != 0;
-        dmax = m ? a + b : 0u;
-        cnt = m ? 1u : 0u;
-        const uint64_t ab = (uint64_t)a * b;
-        const uint64_t aa = (uint64_t)a * a, bb = (uint64_t)b * b;
-        d2 = 2u * ab;
-        d3 = (MT)3u * (MT)ab * (MT)(a + b);
-        d4 = (MT)(2u * ab) * (MT)(2u * aa + 3u * ab + 2u * bb);
-    }
-};
 
 // Running per-run state (after some number of bonds).
 template <bool WIDE>
@@ -791,20 +836,9 @@ __global__ __launch_bounds__(64) void chunk_scan_kernel(
     uint64_t mx = 1, merges = 0, s2 = N;
     MT s3 = N, s4 = N;
     for (uint32_t c = 0; c < C; ++c) {
-        if (lane < PC_STATE_WORDS) {
-            uint64_t w = 0;
-            switch (lane) {
-            case 0: w = mx; break;
-            case 1: w = merges; break;
-            case 2: w = s2; break;
-            case 3: w = (uint64_t)s3; break;
-            case 4: w = (uint64_t)((u128)s3 >> 64); break;
-            case 5: w = (uint64_t)s4; break;
-            case 6: w = (uint64_t)((u128)s4 >> 64); break;
-            default: break;
-            }
-            out[(size_t)c * PC_STATE_WORDS + lane] = w;
-        }
+        if (lane < PC_STATE_WORDS)
+            out[(size_t)c * PC_STATE_WORDS + lane] =
+                state_word(lane, mx, merges, s2, s3, s4);
         const uint32_t q = c * 64u + lane;
         const RecT rec = q < M ? rec_r[q] : (RecT)0;
         const Deltas<WIDE> d(rec);
@@ -1877,7 +1911,7 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
                         int64_t rec_stride, uint32_t *d_span_step,
                         void *d_workspace, uint64_t workspace_bytes,
                         uint32_t merge_rounds, uint32_t state_mode,
-                        void *stream)
+                        uint64_t *d_chunk_state, void *stream)
 {
     const uint32_t rounds = merge_rounds ? merge_rounds : g_merge_rounds;
     if (rounds > 64)
@@ -1907,10 +1941,16 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     if (use_lds) {
         const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
                            (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
-        sweep_lds_kernel<uint32_t><<<num_runs, 64, lds, s>>>(
-            num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
-            d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
-            d_span_step, rounds);
+        if (d_chunk_state)
+            sweep_lds_kernel<uint32_t, true><<<num_runs, 64, lds, s>>>(
+                num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
+                d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
+                d_span_step, rounds, d_chunk_state);
+        else
+            sweep_lds_kernel<uint32_t, false><<<num_runs, 64, lds, s>>>(
+                num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
+                d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
+                d_span_step, rounds, nullptr);
         PC_LAUNCH_CHECK("sweep_lds_kernel");
         return PC_OK;
     }
@@ -1926,17 +1966,20 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         slots = num_runs;
     if (slots > kGlobalSlots)
         slots = kGlobalSlots;
-    if (u32rec) {
-        sweep_global_kernel<uint32_t><<<(uint32_t)slots, 64, 0, s>>>(
-            num_nodes, num_edges, num_runs, edges, d_node_sides,
-            span_on_first_merge, d_perm, perm_stride, (uint32_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace, rounds);
-    } else {
-        sweep_global_kernel<uint64_t><<<(uint32_t)slots, 64, 0, s>>>(
-            num_nodes, num_edges, num_runs, edges, d_node_sides,
-            span_on_first_merge, d_perm, perm_stride, (uint64_t *)d_records,
-            rec_stride, d_span_step, (uint32_t *)d_workspace, rounds);
-    }
+#define PC_SWEEP_GLOBAL(RT, CH)                                                \
+    sweep_global_kernel<RT, CH><<<(uint32_t)slots, 64, 0, s>>>(                \
+        num_nodes, num_edges, num_runs, edges, d_node_sides,                   \
+        span_on_first_merge, d_perm, perm_stride, (RT *)d_records, rec_stride, \
+        d_span_step, (uint32_t *)d_workspace, rounds, d_chunk_state)
+    if (u32rec && d_chunk_state)
+        PC_SWEEP_GLOBAL(uint32_t, true);
+    else if (u32rec)
+        PC_SWEEP_GLOBAL(uint32_t, false);
+    else if (d_chunk_state)
+        PC_SWEEP_GLOBAL(uint64_t, true);
+    else
+        PC_SWEEP_GLOBAL(uint64_t, false);
+#undef PC_SWEEP_GLOBAL
     PC_LAUNCH_CHECK("sweep_global_kernel");
     return PC_OK;
 }
@@ -2303,10 +2346,10 @@ extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
         const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
                            (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_lds_kernel<uint32_t>, 64, lds));
+            &blocks, sweep_lds_kernel<uint32_t, false>, 64, lds));
     } else {
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_global_kernel<uint32_t>, 64, 0));
+            &blocks, sweep_global_kernel<uint32_t, false>, 64, 0));
     }
     *blocks_per_cu = blocks;
     return PC_OK;

@@ -51,14 +51,14 @@ def test_host_only_entry_points():
 def test_argument_errors_without_gpu():
     lib = nat.load_library()
     rc = lib.pc_sweep(0, 1, None, None, 0, None, 1, 1, None, 1, None, None, 0, 0, 0,
-                      None)
+                      None, None)
     assert rc == nat.PC_EINVAL
     assert b'num_nodes' in lib.pc_last_error()
     rc = lib.pc_sweep(16, 1, None, None, 0, None, 1, 1, None, 1, None, None, 0, 99,
-                      0, None)
+                      0, None, None)
     assert rc == nat.PC_EINVAL and b'merge_rounds' in lib.pc_last_error()
     rc = lib.pc_sweep(20000, 1, None, None, 0, None, 1, 1, None, 1, None, None, 0,
-                      0, nat.PC_STATE_LDS, None)
+                      0, nat.PC_STATE_LDS, None, None)
     assert rc == nat.PC_EINVAL
     with pytest.raises(ValueError):
         nat.check(rc, 'pc_sweep')

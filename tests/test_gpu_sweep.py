@@ -171,6 +171,20 @@ def test_pipeline_matches_single_launch(gpu):
         assert np.array_equal(s2.cpu().numpy(), ref[1])
 
 
+def test_pipeline_bit_exact_mode_matches_oracle(gpu):
+    """The pipeline with device RandomState permutations (seeds = run
+    index, the L = 16 configuration) gives the oracle's exact sums over
+    numpy's permutations."""
+    from pypercolate_amd._engine import Pipeline
+    eng = BondEngine(_graph.square_lattice(16))
+    g = oracle.percolation_graph_arrays(spanning_2d_grid(16), True)
+    tr = oracle.bond_runs(g, oracle.randomstate_permutations(range(40), g['num_edges']))
+    ref1, ref2 = oracle.microcanonical_sums(tr)
+    pipe = Pipeline(eng, 40, seeds=np.arange(40), sub_batch=16)
+    s1, s2 = host_sums(*pipe.run())
+    assert (s1 == ref1).all() and (s2 == ref2).all()
+
+
 @pytest.mark.parametrize('M', [1, 2, 12, 480, 32512, 70000])
 def test_randomstate_permutations_bit_exact(gpu, M):
     """Device MT19937 + legacy shuffle == numpy RandomState(seed).permutation
@@ -190,3 +204,19 @@ def test_randomstate_kat_3x3(gpu, golden):
     eng = BondEngine(_graph.square_lattice(3))
     got = eng.randomstate_permutations([kat['seed']]).cpu().numpy()[0]
     assert got.tolist() == kat['permutation']
+
+
+@pytest.mark.parametrize('length,mode', [(16, 'auto'), (128, 'auto'), (64, 'hbm'),
+                                         (130, 'auto'), (257, 'auto')])
+def test_sweep_chunk_states_equal_chunk_scan(gpu, length, mode):
+    """The chunk states the sweep writes itself are bit-identical to the
+    separate pass over the records (pc_chunk_scan): LDS state, HBM state
+    with 32-bit records, HBM state with 64-bit records."""
+    from pypercolate_amd import _native as nat
+    eng = BondEngine(_graph.square_lattice(length))
+    if mode == 'hbm':
+        eng.state_mode = nat.PC_STATE_HBM
+    res = eng.sweep(eng.device_permutations(77, 96), chunks=True)
+    fused = res.chunk_state.cpu().numpy()
+    scanned = eng.chunk_scan(res).cpu().numpy()
+    assert np.array_equal(fused, scanned)
