@@ -564,9 +564,17 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 //     both of its slots shares no root with any earlier pending candidate:
 //     no earlier merge can change its clusters, so it merges now, in
 //     parallel with the other such candidates (their roots are distinct).
-//     The others re-find their roots and go to the next round; the rounds
-//     stop early once one frees at most a single candidate.  Hash
-//     collisions only make the test conservative;
+//     Star merges: the root the first blocked candidate is blocked on (the
+//     hub, near and above the threshold usually the largest cluster's root)
+//     takes in, in the same round, every pending candidate touching it, in
+//     step order up to the first whose other root an earlier candidate also
+//     touches, and up to the first other pending candidate that touches a
+//     root the star absorbs; their records follow from an exclusive prefix
+//     sum of the absorbed sizes and the spanning step from two flag
+//     ballots.  The
+//     others re-find their roots and go to the next round; the rounds stop
+//     early once one frees at most a single candidate.  Hash collisions
+//     only make the tests conservative;
 //  3. the candidates left after `rounds` rounds are resolved in step order
 //     with uniform control flow: roots re-read from the state (one address
 //     for the whole wave, made uniform with readfirstlane) and followed
@@ -628,6 +636,12 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         RecT rec = 0;
 
         for (uint32_t k = 0; k < rounds && pending; ++k) {
+            if (tag >= 0xFFFFFF80u) {
+                // the round tags would wrap: clear the table and restart
+                for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
+                    claims[i] = 0u;
+                tag = 0u;
+            }
             tag += 64u;
             const uint32_t mine = tag + 63u - lane;
             const uint32_t su = claim_slot(ru), sv = claim_slot(rv);
@@ -635,8 +649,71 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 atomicMax(&claims[su], mine);
                 atomicMax(&claims[sv], mine);
             }
-            const bool go = cand && claims[su] == mine && claims[sv] == mine;
-            const uint64_t freed = __ballot(go);
+            const bool own_u = claims[su] == mine, own_v = claims[sv] == mine;
+            bool go = cand && own_u && own_v;
+            // star: the pending candidates that touch one hub root (the root
+            // the first blocked candidate is blocked on, usually the largest
+            // cluster's), in step order up to the first whose other root an
+            // earlier candidate also touches, all merge into the hub now
+            uint64_t star = 0;
+            uint32_t hub = 0;
+            const uint64_t blocked = __ballot(cand && !go);
+            if (blocked) {
+                const uint32_t fl = (uint32_t)__builtin_ctzll(blocked);
+                hub = read_lane(own_u ? rv : ru, fl);
+                const bool at_u = ru == hub;
+                const bool touch = cand && (at_u || rv == hub);
+                const uint64_t tm = __ballot(touch);
+                const uint64_t bad = tm & ~__ballot(touch && (at_u ? own_v : own_u));
+                star = bad ? (tm & ((bad & (0u - bad)) - 1u)) : tm;
+                go = go && !touch;
+                // a pending candidate outside the star whose root's slot is
+                // held by a member touches a root the star absorbs before
+                // its step: the star stops before it (the slot holder is
+                // the earliest claimant, lane tag + 63 - claims[slot])
+                const uint32_t hu = tag + 63u - claims[su], hv = tag + 63u - claims[sv];
+                const bool late = cand && !go && !((star >> lane) & 1u) &&
+                                  (((star >> (hu & 63u)) & 1u) || ((star >> (hv & 63u)) & 1u));
+                const uint64_t lm = __ballot(late);
+                if (lm)
+                    star &= (lm & (0u - lm)) - 1u;
+            }
+            const bool in_star = (star >> lane) & 1u;
+            const uint64_t freed = __ballot(go) | star;
+            if (star) {
+                // records in step order: the hub's size grows by the sizes
+                // absorbed by the earlier members (exclusive prefix sum)
+                const bool at_u = ru == hub;
+                const uint32_t x = at_u ? rv : ru;
+                uint32_t sx, fx, sh, fh;
+                st.decode(at_u ? cv : cu, sx, fx);
+                st.decode(at_u ? cu : cv, sh, fh);
+                const uint32_t first = (uint32_t)__builtin_ctzll(star);
+                const uint32_t s_hub = read_lane(sh, first), f_hub = read_lane(fh, first);
+                const uint32_t mine_sx = in_star ? sx : 0u;
+                const uint32_t incl = wave_scan_add(mine_sx);
+                const uint32_t total = read_lane(incl, 63u);
+                const uint64_t m1 = __ballot(in_star && (fx & 1u));
+                const uint64_t m2 = __ballot(in_star && (fx & 2u));
+                const uint32_t f_all = f_hub | (m1 ? 1u : 0u) | (m2 ? 2u : 0u);
+                if (in_star) {
+                    // (size on u's side, size on v's side), as every merge
+                    st.set(x, hub);
+                    const uint32_t sh_now = s_hub + incl - sx;
+                    rec = at_u ? pack_record<RecT>(sh_now, sx)
+                               : pack_record<RecT>(sx, sh_now);
+                }
+                if (lane == first)
+                    st.set(hub, st.root(s_hub + total, f_all == 3u ? 0u : f_all));
+                if (span_first)
+                    span_at = min(span_at, base + first + 1u);
+                if (f_all == 3u) {
+                    // first member after which both sides are touched
+                    const uint32_t l1 = (f_hub & 1u) ? first : (uint32_t)__builtin_ctzll(m1);
+                    const uint32_t l2 = (f_hub & 2u) ? first : (uint32_t)__builtin_ctzll(m2);
+                    span_at = min(span_at, base + max(l1, l2) + 1u);
+                }
+            }
             if (go) {
                 uint32_t sa, fa, sb, fb;
                 st.decode(cu, sa, fa);
@@ -656,7 +733,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 if (spans)
                     span_at = min(span_at, base + (uint32_t)__builtin_ctzll(spans) + 1u);
             }
-            cand = cand && !go;
+            cand = cand && !go && !in_star;
             st.sync();
             if (__ballot(cand)) {
                 // lanes still pending follow their roots past this round's merges
