@@ -57,6 +57,9 @@ def parse_args(argv=None):
                     help='pick the sweep knobs with the autotuner first (the '
                          'default knobs are within 1%% of the tuned ones at '
                          'L = 128; the finite-size batch always tunes)')
+    ap.add_argument('--bitexact', action='store_true',
+                    help='also time the pipeline with bit-exact device '
+                         'RandomState permutations')
     ap.add_argument('--batch', action='store_true',
                     help='time the finite-size batch (L = 16..512, 2^26/L^2 '
                          'runs each, one HIP graph) instead of L = 128')
@@ -384,6 +387,32 @@ def reference_class_baseline(length, single_runs, runs_per_proc):
     }
 
 
+def bitexact_mode(eng, R, steps, start, stop):
+    """The L = 128 pipeline with numpy RandomState(r).permutation(M) for run r
+    drawn on the device (MT19937 + the legacy shuffle)."""
+    import torch
+    from pypercolate_amd._engine import Pipeline
+    exact = Pipeline(eng, R, seeds=np.arange(R))
+    exact.run()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(steps):
+        exact.run()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    s1 = exact.sum1.cpu().numpy().view('uint64')
+    res = {'value': float(R) * eng.M / (ms / 1e3), 'unit': 'bond additions/s',
+           'ms_per_step': ms,
+           'permutations': 'numpy RandomState(r).permutation(M), r = 0..{}, '
+                           'on the device (MT19937 + legacy shuffle)'.format(R - 1),
+           'sums_check': bool(int(s1[0, eng.M, 0]) == R * eng.N and
+                              int(s1[1, eng.M, 0]) == R)}
+    del exact
+    torch.cuda.empty_cache()
+    return res
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -495,26 +524,12 @@ def main(argv=None):
     }
     del pipe
     torch.cuda.empty_cache()
-    # bit-exact mode: numpy RandomState(r).permutation(M) for run r, drawn on
-    # the device (MT19937 + the legacy shuffle), same pipeline otherwise
-    exact = Pipeline(eng, R, seeds=np.arange(R))
-    exact.run()
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(args.steps):
-        exact.run()
-    stop.record()
-    torch.cuda.synchronize()
-    ex_ms = start.elapsed_time(stop) / args.steps
-    ex1 = exact.sum1.cpu().numpy().view('uint64')
-    out['bitexact_mode'] = {
-        'value': adds / (ex_ms / 1e3), 'unit': 'bond additions/s',
-        'ms_per_step': ex_ms,
-        'permutations': 'numpy RandomState(r).permutation(M), r = 0..{}, '
-                        'on the device (MT19937 + legacy shuffle)'.format(R - 1),
-        'sums_check': bool(int(ex1[0, M, 0]) == R * N and int(ex1[1, M, 0]) == R)}
-    del exact
-    torch.cuda.empty_cache()
+    # bit-exact mode (--bitexact): numpy RandomState(r).permutation(M) for
+    # run r, drawn on the device (MT19937 + the legacy shuffle), same
+    # pipeline otherwise.  Opt-in: it launches the same sweep kernel, which
+    # would mix into the kernel statistics of the Philox pipeline.
+    if args.bitexact:
+        out['bitexact_mode'] = bitexact_mode(eng, R, args.steps, start, stop)
     if args.canonical_length > 0:
         info, big, sums, runs = large_lattice(args.canonical_length,
                                               args.canonical_runs, seed=7)

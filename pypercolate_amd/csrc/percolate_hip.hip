@@ -897,33 +897,83 @@ __device__ __forceinline__ void observables(const RunState<WIDE> &st,
 }
 
 // ---------------------------------------------------------------------------
-// chunk_scan: one wavefront per run walks its records 64 at a time.
+// chunk_scan in two passes.  (A) chunk_delta: every (run, chunk) in
+// parallel, the totals of the chunk's deltas {max merged size, merges, dS2,
+// dS3, dS4} reduced across the wave into that chunk's entry.  (B)
+// chunk_prefix: one wavefront per run turns the entries in place into the
+// state at the start of each chunk, 64 chunks per step (scans across lanes,
+// a carry between steps).
 // ---------------------------------------------------------------------------
+#define CHUNK_DELTA_PER_BLOCK 8u
+
 template <typename RecT, bool WIDE>
-__global__ __launch_bounds__(64) void chunk_scan_kernel(
-    uint32_t N, uint32_t M, const RecT *__restrict__ records,
-    int64_t rec_stride, uint64_t *__restrict__ chunk_state)
+__global__ __launch_bounds__(64) void chunk_delta_kernel(
+    uint32_t M, const RecT *__restrict__ records, int64_t rec_stride,
+    uint64_t *__restrict__ chunk_state)
+{
+    // blocks: run-major, CHUNK_DELTA_PER_BLOCK chunks each
+    const uint32_t C = (M + 63u) / 64u;
+    const uint32_t per_run = (C + CHUNK_DELTA_PER_BLOCK - 1u) / CHUNK_DELTA_PER_BLOCK;
+    const uint32_t r = blockIdx.x / per_run, lane = threadIdx.x;
+    const RecT *rec_r = records + (int64_t)r * rec_stride;
+    uint64_t *out = chunk_state + (size_t)r * C * PC_STATE_WORDS;
+    const uint32_t c0 = (blockIdx.x % per_run) * CHUNK_DELTA_PER_BLOCK;
+    const uint32_t c1 = min(C, c0 + CHUNK_DELTA_PER_BLOCK);
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t q = c * 64u + lane;
+        const Deltas<WIDE> d(q < M ? rec_r[q] : (RecT)0);
+        const uint64_t dmax = read_lane(wave_scan_max(d.dmax), 63u);
+        const uint64_t cnt = read_lane(wave_scan_add(d.cnt), 63u);
+        const uint64_t d2 = read_lane(wave_scan_add(d.d2), 63u);
+        const auto d3 = read_lane(wave_scan_add(d.d3), 63u);
+        const auto d4 = read_lane(wave_scan_add(d.d4), 63u);
+        if (lane < PC_STATE_WORDS)
+            out[(size_t)c * PC_STATE_WORDS + lane] = state_word(lane, dmax, cnt, d2, d3, d4);
+    }
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(64) void chunk_prefix_kernel(
+    uint32_t N, uint32_t M, uint64_t *__restrict__ chunk_state)
 {
     using MT = typename Wide<WIDE>::T;
     const uint32_t r = blockIdx.x, lane = threadIdx.x;
     const uint32_t C = (M + 63u) / 64u;
-    const RecT *rec_r = records + (int64_t)r * rec_stride;
-    uint64_t *out = chunk_state + (size_t)r * C * PC_STATE_WORDS;
-
+    uint64_t *st = chunk_state + (size_t)r * C * PC_STATE_WORDS;
+    // state at the start of the current group of 64 chunks
     uint64_t mx = 1, merges = 0, s2 = N;
     MT s3 = N, s4 = N;
-    for (uint32_t c = 0; c < C; ++c) {
-        if (lane < PC_STATE_WORDS)
-            out[(size_t)c * PC_STATE_WORDS + lane] =
-                state_word(lane, mx, merges, s2, s3, s4);
-        const uint32_t q = c * 64u + lane;
-        const RecT rec = q < M ? rec_r[q] : (RecT)0;
-        const Deltas<WIDE> d(rec);
-        mx = umax64(mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
-        merges += read_lane(wave_scan_add(d.cnt), 63u);
-        s2 += read_lane(wave_scan_add(d.d2), 63u);
-        s3 += read_lane(wave_scan_add(d.d3), 63u);
-        s4 += read_lane(wave_scan_add(d.d4), 63u);
+    for (uint32_t c0 = 0; c0 < C; c0 += 64u) {
+        const uint32_t c = c0 + lane;
+        uint64_t w[PC_STATE_WORDS] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (c < C)
+            for (uint32_t k = 0; k < PC_STATE_WORDS; ++k)
+                w[k] = st[(size_t)c * PC_STATE_WORDS + k];
+        const uint32_t dmax = (uint32_t)w[0];
+        const uint64_t cnt = w[1], d2 = w[2];
+        MT d3 = (MT)w[3], d4 = (MT)w[5];
+        if constexpr (WIDE) {
+            d3 |= (MT)w[4] << 64;
+            d4 |= (MT)w[6] << 64;
+        }
+        const uint32_t imax = wave_scan_max(dmax);
+        const uint64_t icnt = wave_scan_add(cnt), id2 = wave_scan_add(d2);
+        const MT id3 = wave_scan_add(d3), id4 = wave_scan_add(d4);
+        // exclusive prefix: the sums minus the own chunk, the maximum of the
+        // chunks before (the previous lane's inclusive maximum)
+        uint32_t emax = (uint32_t)__shfl_up((int)imax, 1, 64);
+        if (lane == 0)
+            emax = 0u;
+        if (c < C)
+            for (uint32_t k = 0; k < PC_STATE_WORDS; ++k)
+                st[(size_t)c * PC_STATE_WORDS + k] = state_word(
+                    k, umax64(mx, emax), merges + icnt - cnt, s2 + id2 - d2,
+                    s3 + id3 - d3, s4 + id4 - d4);
+        mx = umax64(mx, read_lane(imax, 63u));
+        merges += read_lane(icnt, 63u);
+        s2 += read_lane(id2, 63u);
+        s3 += read_lane(id3, 63u);
+        s4 += read_lane(id4, 63u);
     }
 }
 
@@ -2074,19 +2124,29 @@ extern "C" int pc_chunk_scan(uint32_t num_nodes, uint32_t num_edges,
     if (!d_records || !d_chunk_state || rec_stride < (int64_t)num_edges)
         return pc_fail(PC_EINVAL, "pc_chunk_scan: bad buffer");
     hipStream_t s = as_stream(stream);
+    const uint32_t C = (num_edges + 63u) / 64u;
+    const uint64_t blocks = (uint64_t)((C + CHUNK_DELTA_PER_BLOCK - 1u) / CHUNK_DELTA_PER_BLOCK) *
+                            num_runs;
+    if (blocks > 0x7FFFFFFFu)
+        return pc_fail(PC_EOVERFLOW, "pc_chunk_scan: too many runs x chunks in one call");
+    const dim3 grid((uint32_t)blocks);
     if (num_nodes <= kLdsMaxNodes)
-        chunk_scan_kernel<uint32_t, false><<<num_runs, 64, 0, s>>>(
-            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
-            d_chunk_state);
+        chunk_delta_kernel<uint32_t, false><<<grid, 64, 0, s>>>(
+            num_edges, (const uint32_t *)d_records, rec_stride, d_chunk_state);
     else if (num_nodes <= kU32RecordMaxNodes)
-        chunk_scan_kernel<uint32_t, true><<<num_runs, 64, 0, s>>>(
-            num_nodes, num_edges, (const uint32_t *)d_records, rec_stride,
-            d_chunk_state);
+        chunk_delta_kernel<uint32_t, true><<<grid, 64, 0, s>>>(
+            num_edges, (const uint32_t *)d_records, rec_stride, d_chunk_state);
     else
-        chunk_scan_kernel<uint64_t, true><<<num_runs, 64, 0, s>>>(
-            num_nodes, num_edges, (const uint64_t *)d_records, rec_stride,
-            d_chunk_state);
-    PC_LAUNCH_CHECK("chunk_scan_kernel");
+        chunk_delta_kernel<uint64_t, true><<<grid, 64, 0, s>>>(
+            num_edges, (const uint64_t *)d_records, rec_stride, d_chunk_state);
+    PC_LAUNCH_CHECK("chunk_delta_kernel");
+    if (num_nodes <= kLdsMaxNodes)
+        chunk_prefix_kernel<false><<<num_runs, 64, 0, s>>>(num_nodes, num_edges,
+                                                           d_chunk_state);
+    else
+        chunk_prefix_kernel<true><<<num_runs, 64, 0, s>>>(num_nodes, num_edges,
+                                                          d_chunk_state);
+    PC_LAUNCH_CHECK("chunk_prefix_kernel");
     return PC_OK;
 }
 
