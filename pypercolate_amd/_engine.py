@@ -267,8 +267,10 @@ class Pipeline(object):
         dev = eng.device
         M, S = eng.M, eng.S
         if sub_batch is None:
+            # a whole number of residencies (a partial one leaves CUs idle at
+            # the end of every launch); three measured best at L = 128
             cus = torch.cuda.get_device_properties(dev).multi_processor_count
-            sub_batch = 2 * max(eng.occupancy(), 1) * cus
+            sub_batch = 3 * max(eng.occupancy(), 1) * cus
         self.sub = max(1, min(int(sub_batch), self.R))
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
         self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
