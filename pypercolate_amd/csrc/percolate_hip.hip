@@ -622,6 +622,13 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     int2 uv_next = make_int2(0, 0);
     if (lane < M)
         uv_next = edges[perm[lane]];
+    // codes of the next batch's endpoints, read one batch early: parent
+    // links only ever move up the tree (a union links a root under a root,
+    // halving replaces a parent by its grandparent), so a code read before
+    // this batch's merges still names an ancestor and the next find may
+    // start there, one dependent read shorter.  A root code is no hint
+    // (the node may have been linked since): that find reads afresh.
+    uint32_t hint_u = 0xFFFFFFFFu, hint_v = 0xFFFFFFFFu;
 
     for (uint32_t base = 0; base < M; base += 64u) {
         const uint32_t n = base + lane;
@@ -633,8 +640,12 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
         e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
 
-        uint32_t ru = (uint32_t)uv.x, rv = (uint32_t)uv.y, cu, cv;
+        uint32_t ru = st.is_root(hint_u) ? (uint32_t)uv.x : hint_u;
+        uint32_t rv = st.is_root(hint_v) ? (uint32_t)uv.y : hint_v;
+        uint32_t cu, cv;
         find2(st, ru, cu, rv, cv);
+        hint_u = st.get((uint32_t)uv_next.x);
+        hint_v = st.get((uint32_t)uv_next.y);
         bool cand = active && ru != rv;
         uint64_t pending = __ballot(cand);
         RecT rec = 0;
@@ -2196,6 +2207,10 @@ extern "C" int pc_microcanonical_sums(uint32_t num_nodes, uint32_t num_edges,
     uint32_t groups = (8192u + C - 1u) / C;
     if (groups > num_runs)
         groups = num_runs;
+    // at most 2^16 runs per group: the narrow 128-bit sum of squares of the
+    // fourth moment (each below 2^112) stays exact
+    if ((num_runs + groups - 1u) / groups > 65536u)
+        groups = (num_runs + 65535u) / 65536u;
     const uint32_t rpg = (num_runs + groups - 1u) / groups;
     groups = (num_runs + rpg - 1u) / rpg;
     const dim3 grid(C, groups);
