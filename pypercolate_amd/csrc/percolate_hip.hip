@@ -1578,21 +1578,6 @@ __device__ __forceinline__ void walk_init(BinomWalk &b, const FusedCtx &f,
     b.w = 0.0;
 }
 
-// Weight at k (k advances by one per call); `fresh` replaces the ratio step
-// by a direct evaluation (a uniform branch every FUSED_REANCHOR steps).
-__device__ __forceinline__ double walk_step(BinomWalk &b, int64_t k, bool fresh)
-{
-    const double kd = (double)k;
-    double v;
-    if (fresh && k > b.ka && k < b.kend)
-        v = binom_pmf_outline(kd, b.n, b.p);
-    else
-        v = b.w * ((b.n - kd + 1.0) / kd) * b.pq;
-    v = (k == b.ka) ? b.anchor : v;
-    b.w = v;
-    return (k >= b.ka && k < b.kend) ? v : 0.0;
-}
-
 // Row tile of workgroup b out of T: workgroups are dealt to the 8 XCDs in
 // turn, so give each XCD a contiguous run of tiles; neighbouring p-values
 // read the same rows of X and then share that XCD's L2.
@@ -1601,6 +1586,22 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
     const uint32_t x = b % 8u, idx = b / 8u;
     const uint32_t base = T / 8u, extra = T % 8u;
     return x * base + min(x, extra) + idx;
+}
+
+// Weight at k (k advances by one per call) from the weight at k - 1 and
+// ratio = (n - k + 1) / k; `fresh` replaces the ratio step by a direct
+// evaluation (a uniform branch every FUSED_REANCHOR steps).
+__device__ __forceinline__ double walk_step(BinomWalk &b, int64_t k, bool fresh,
+                                            double ratio)
+{
+    double v;
+    if (fresh && k > b.ka && k < b.kend)
+        v = binom_pmf_outline((double)k, b.n, b.p);
+    else
+        v = b.w * ratio * b.pq;
+    v = (k == b.ka) ? b.anchor : v;
+    b.w = v;
+    return (k >= b.ka && k < b.kend) ? v : 0.0;
 }
 
 __device__ __forceinline__ double fused_x(const FusedCtx &f, int64_t k, uint32_t c)
@@ -1655,17 +1656,36 @@ __global__ __launch_bounds__(64) void canonical_fused_f64_kernel(
         const int64_t kq1 = min(kq0 + Lq, g_end);
         BinomWalk b;
         walk_init(b, f, p0 + i, kq0, kq1);
+        // the ratio (n - k + 1) / k depends on k only, shared by the 16 rows
+        // of a lane group: each lane divides once per 16 steps, for step
+        // j0 + lane % 16 of group lane / 16, and the steps read it back
+        // with a lane shuffle
+        double rr = 0.0;
+        // columns one chunk ahead
+        double xn[FUSED_F64_UNROLL];
+#pragma unroll
+        for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
+            const int64_t k = kq0 + u;
+            xn[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+        }
         for (int64_t j0 = 0; j0 < Lq; j0 += FUSED_F64_UNROLL) {
             double a[FUSED_F64_UNROLL], x[FUSED_F64_UNROLL];
 #pragma unroll
             for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
-                const int64_t k = kq0 + j0 + u;
-                x[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+                x[u] = xn[u];
+                const int64_t k = kq0 + j0 + FUSED_F64_UNROLL + u;
+                xn[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+            }
+            if ((j0 & 15) == 0) {
+                const int64_t kr = g_begin + (int64_t)(lane / 16u) * Lq + j0 + (lane % 16u);
+                rr = (b.n - (double)kr + 1.0) / (double)kr;
             }
             const bool fresh = j0 > 0 && j0 % FUSED_REANCHOR == 0;
 #pragma unroll
-            for (int u = 0; u < FUSED_F64_UNROLL; ++u)
-                a[u] = walk_step(b, kq0 + j0 + u, fresh && u == 0);
+            for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
+                const double ratio = __shfl(rr, (int)(q * 16u) + (int)(j0 & 15) + u, 64);
+                a[u] = walk_step(b, kq0 + j0 + u, fresh && u == 0, ratio);
+            }
 #pragma unroll
             for (int u = 0; u < FUSED_F64_UNROLL; u += 2) {
                 acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], x[u], acc0, 0, 0, 0);
@@ -1719,10 +1739,16 @@ __global__ __launch_bounds__(64) void canonical_fused_split_kernel(
                 m = fmax(m, fabs(x[e]));
             }
             const bool fresh = j0 > 0 && j0 % FUSED_REANCHOR == 0;
+            // one division per lane per window: the ratio (n - k + 1) / k
+            // for step j0 + lane % 16 of group lane / 16, read back by the
+            // 16 rows of each group with a lane shuffle
+            const int64_t kr = g_begin + (int64_t)(lane / 16u) * Lq + j0 + (lane % 16u);
+            const double rr = (b.n - (double)kr + 1.0) / (double)kr;
             double wm = 0.0;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                w[e] = walk_step(b, kq0 + j0 + e, fresh && e == 0);
+                const double ratio = __shfl(rr, (int)(q * 16u) + e, 64);
+                w[e] = walk_step(b, kq0 + j0 + e, fresh && e == 0, ratio);
                 wm = fmax(wm, w[e]);
             }
             // scales of column i and row i over the window: lanes i, i+16,
