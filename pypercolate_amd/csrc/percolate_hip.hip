@@ -1528,6 +1528,9 @@ __global__ __launch_bounds__(256) void f64_to_f32_kernel(
 // ---------------------------------------------------------------------------
 #define FUSED_REANCHOR 1024
 #define FUSED_F64_UNROLL 8
+#ifndef FUSED_F64_AHEAD
+#define FUSED_F64_AHEAD 1
+#endif
 
 struct FusedCtx {
     const double *x;
@@ -1661,20 +1664,26 @@ __global__ __launch_bounds__(64) void canonical_fused_f64_kernel(
         // j0 + lane % 16 of group lane / 16, and the steps read it back
         // with a lane shuffle
         double rr = 0.0;
-        // columns one chunk ahead
-        double xn[FUSED_F64_UNROLL];
+        // columns FUSED_F64_AHEAD chunks ahead (a miss to the MALL or HBM
+        // outlasts one chunk of MFMAs several times over)
+        double xq[FUSED_F64_AHEAD][FUSED_F64_UNROLL];
 #pragma unroll
-        for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
-            const int64_t k = kq0 + u;
-            xn[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
-        }
+        for (int d = 0; d < FUSED_F64_AHEAD; ++d)
+#pragma unroll
+            for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
+                const int64_t k = kq0 + d * FUSED_F64_UNROLL + u;
+                xq[d][u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+            }
         for (int64_t j0 = 0; j0 < Lq; j0 += FUSED_F64_UNROLL) {
             double a[FUSED_F64_UNROLL], x[FUSED_F64_UNROLL];
 #pragma unroll
             for (int u = 0; u < FUSED_F64_UNROLL; ++u) {
-                x[u] = xn[u];
-                const int64_t k = kq0 + j0 + FUSED_F64_UNROLL + u;
-                xn[u] = k < kq1 ? fused_x(f, k, c) : 0.0;
+                x[u] = xq[0][u];
+#pragma unroll
+                for (int d = 0; d + 1 < FUSED_F64_AHEAD; ++d)
+                    xq[d][u] = xq[d + 1][u];
+                const int64_t k = kq0 + j0 + FUSED_F64_AHEAD * FUSED_F64_UNROLL + u;
+                xq[FUSED_F64_AHEAD - 1][u] = k < kq1 ? fused_x(f, k, c) : 0.0;
             }
             if ((j0 & 15) == 0) {
                 const int64_t kr = g_begin + (int64_t)(lane / 16u) * Lq + j0 + (lane % 16u);
@@ -2339,10 +2348,13 @@ extern "C" int pc_binomial_weights(uint32_t n, uint32_t num_p,
 
 // K-slices of the MFMA contraction: enough tiles x slices to fill the GPU
 // with about 16k wavefronts, at most PC_CONTRACT_SLICES.
+#ifndef PC_CONTRACT_WAVES
+#define PC_CONTRACT_WAVES 16384u
+#endif
 static uint32_t contract_slices(uint32_t num_p, uint32_t cols)
 {
     const uint64_t tiles = (uint64_t)((num_p + 15u) / 16u) * ((cols + 15u) / 16u);
-    uint64_t sl = (16384u + tiles - 1u) / (tiles ? tiles : 1u);
+    uint64_t sl = (PC_CONTRACT_WAVES + tiles - 1u) / (tiles ? tiles : 1u);
     if (sl < 1u)
         sl = 1u;
     if (sl > PC_CONTRACT_SLICES)
