@@ -3,10 +3,13 @@
 // include/percolate_hip.h.
 //
 // Pipeline for R runs on a graph with N nodes and M edges:
-//   permute (Philox-keyed Feistel, optional) -> sweep (union-find, one run per
-//   wavefront, merge records) -> chunk_scan (per-run state every 64 steps) ->
+//   permute (Philox-keyed Feistel, or MT19937 + numpy's shuffle for
+//   bit-exact RandomState permutations) -> sweep (union-find, one run per
+//   wavefront, merge records) -> chunk states (per-run state every 64
+//   steps: two passes over the records, or written by the sweep itself) ->
 //   microcanonical sums (exact Sigma x, Sigma x^2 over runs per step) or
-//   expand (per-run traces).
+//   expand (per-run traces) -> canonical averages (binomial convolution on
+//   the matrix cores, the weights generated inside the contraction).
 //
 // The observables of one run at step s (largest cluster, spanning flag,
 // moments k = 0..4 of all clusters but the largest) depend only on the set
