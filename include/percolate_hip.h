@@ -260,6 +260,25 @@ int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
                               const uint64_t *d_sum1, const uint64_t *d_sum2,
                               double *d_mean, double *d_std, void *stream);
 
+/*
+ * The microcanonical statistics of every step with their confidence bounds,
+ * from pc_microcanonical_sums: the epilogue of percolate/percolate.py:557-705
+ * (_microcanonical_average_spanning_cluster, _max_cluster_size, _moments)
+ * on the device.  d_out[c * S + s], S = num_edges + 1, 21 columns:
+ * 0..2 max cluster size mean, lower, upper bound; 3..5 spanning posterior
+ * mean (k + 1) / (R + 2) and the bounds d_beta[2k], d_beta[2k + 1]
+ * (k = runs with a spanning cluster; d_beta holds the caller's beta
+ * quantiles for k = 0..R); 6 + 3j .. 8 + 3j moment j = 0..4 mean, lower,
+ * upper bound.  A bound is mean + (std / sqrt(R)) * t_lo (t_hi), rounded
+ * like the host's scale * ppf + loc, where R > 1 and std > 0, else the
+ * mean; t_lo, t_hi are the caller's Student-t quantiles (R - 1 degrees of
+ * freedom) of the interval.
+ */
+int pc_microcanonical_ci(uint32_t num_edges, uint32_t num_runs,
+                         const uint64_t *d_sum1, const uint64_t *d_sum2,
+                         double t_lo, double t_hi, const double *d_beta,
+                         double *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -536,6 +536,35 @@ def canonical_fused(x, ps, precision='fp64_mfma'):
     return FusedPlan(x, ps, precision).run()
 
 
+def micro_ci(sum1, sum2, num_edges, num_runs, alpha):
+    """The microcanonical statistics of every step with their confidence
+    bounds, on the device (pc_microcanonical_ci): (21, S) float64 tensor,
+    rows 0..2 max cluster size mean / lower / upper, 3..5 spanning, 6 + 3j
+    .. 8 + 3j moment j.  The quantiles come from scipy on the host: Student's
+    t at the interval's two ends (R - 1 degrees of freedom) and the beta
+    quantiles of the spanning counts that occur."""
+    import scipy.stats
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    S = num_edges + 1
+    t_lo = t_hi = 0.0
+    if num_runs > 1:
+        t_lo, t_hi = scipy.stats.t.interval(1 - alpha, df=num_runs - 1)
+    k = np.unique(sum1[1, :, 0].cpu().numpy().view(np.uint64))
+    beta = np.full((num_runs + 1, 2), np.nan)
+    q = scipy.stats.beta.ppf(np.array([alpha / 2, 1 - alpha / 2])[:, None],
+                             k + 1, num_runs - k + 1)
+    beta[k, 0] = q[0]
+    beta[k, 1] = q[1]
+    d_beta = torch.from_numpy(beta).to(sum1.device)
+    out = torch.empty((21, S), dtype=torch.float64, device=sum1.device)
+    nat.check(lib.pc_microcanonical_ci(
+        num_edges, num_runs, nat.ptr(sum1), nat.ptr(sum2), float(t_lo),
+        float(t_hi), nat.ptr(d_beta), nat.ptr(out), nat.stream_handle(torch)),
+        'pc_microcanonical_ci')
+    return out
+
+
 def micro_moments(sum1, sum2, num_edges, num_runs):
     """Mean and sample std (ddof=1) per column and step: (7, S) float64."""
     torch = nat.torch_cuda()

@@ -66,6 +66,9 @@ SIGNATURES = {
                                                          ctypes.POINTER(_u64)]),
     'pc_canonical_fused': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32, _vp, _vp,
                                           _vp, _vp, _i32, _vp, _u64, _vp]),
+    'pc_microcanonical_ci': (ctypes.c_int, [_u32, _u32, _vp, _vp,
+                                            ctypes.c_double, ctypes.c_double,
+                                            _vp, _vp, _vp]),
     'pc_convert_f64_to_f32': (ctypes.c_int, [_vp, _i64, _vp, _vp]),
     'pc_window_traces': (ctypes.c_int, [_u32, _u32, _u32, _vp, _i64, _vp, _vp,
                                         _i64, _i64, _i32, _vp, _vp]),

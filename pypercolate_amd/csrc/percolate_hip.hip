@@ -2016,6 +2016,67 @@ __global__ __launch_bounds__(256) void micro_moments_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Microcanonical statistics with their confidence bounds (the epilogue of
+// percolate/percolate.py:557-705) for every step from the exact sums.
+// Thread = step s; out[c * S + s] for the 21 columns
+//   0..2   max cluster size: mean, lower, upper bound,
+//   3..5   spanning: (k + 1) / (R + 2) and the bounds beta[2k], beta[2k + 1]
+//          (k = runs with a spanning cluster at s),
+//   6 + 3j, 7 + 3j, 8 + 3j   moment j = 0..4: mean, lower, upper bound,
+// a bound being mean + (std / sqrt(R)) * t_lo (t_hi) where R > 1 and
+// std > 0, the mean otherwise.  Products and sums are rounded one by one
+// (no fused multiply-add), as the host's scale * ppf + loc.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mean_std(const uint64_t *__restrict__ sum1,
+                                         const uint64_t *__restrict__ sum2,
+                                         uint64_t i, uint64_t R, double &mean,
+                                         double &std)
+{
+    const uint64_t lo = sum1[2 * i], hi = sum1[2 * i + 1];
+    mean = u128_div_to_double(lo, hi, R);
+    if (R < 2) {
+        std = __longlong_as_double(0x7FF8000000000000ll); // NaN, as ddof=1
+        return;
+    }
+    const uint64_t s2[3] = {sum2[3 * i], sum2[3 * i + 1], sum2[3 * i + 2]};
+    const u256 d = sub_u256(mul_u192_u64(s2, R), square_u128(lo, hi));
+    std = sqrt(u256_to_double(d) / ((double)R * (double)(R - 1)));
+}
+
+__global__ __launch_bounds__(256) void micro_ci_kernel(
+    uint64_t S, uint64_t R, const uint64_t *__restrict__ sum1,
+    const uint64_t *__restrict__ sum2, double t_lo, double t_hi,
+    const double *__restrict__ beta, double *__restrict__ out)
+{
+    // scale * t + mean rounded twice, never fused (HIP contracts by default)
+#pragma clang fp contract(off)
+    const uint64_t s = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= S)
+        return;
+    const double sqrt_r = sqrt((double)R);
+    // columns of the sums: 0 max, 1 spanning, 2..6 moments 0..4
+    const int src[6] = {0, 2, 3, 4, 5, 6};
+    const int dst[6] = {0, 6, 9, 12, 15, 18};
+    for (int j = 0; j < 6; ++j) {
+        double m, sd;
+        mean_std(sum1, sum2, (uint64_t)src[j] * S + s, R, m, sd);
+        double lo = m, hi = m;
+        if (R > 1 && sd != 0.0) {
+            const double scale = sd / sqrt_r;
+            lo = t_lo * scale + m;
+            hi = t_hi * scale + m;
+        }
+        out[(uint64_t)dst[j] * S + s] = m;
+        out[(uint64_t)(dst[j] + 1) * S + s] = lo;
+        out[(uint64_t)(dst[j] + 2) * S + s] = hi;
+    }
+    const uint64_t k = sum1[2 * (S + s)]; // spanning runs (< 2^64)
+    out[3 * S + s] = ((double)k + 1.0) / ((double)R + 2.0);
+    out[4 * S + s] = beta[2 * k];
+    out[5 * S + s] = beta[2 * k + 1];
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 static const uint32_t kLdsMaxNodes = 16384u;
@@ -2508,6 +2569,23 @@ extern "C" int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out
         return pc_fail(PC_EINVAL, "pc_convert_f64_to_f32: NULL buffer");
     f64_to_f32_kernel<<<2048, 256, 0, as_stream(stream)>>>(d_in, n, d_out);
     PC_LAUNCH_CHECK("f64_to_f32_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_microcanonical_ci(uint32_t num_edges, uint32_t num_runs,
+                                    const uint64_t *d_sum1, const uint64_t *d_sum2,
+                                    double t_lo, double t_hi, const double *d_beta,
+                                    double *d_out, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_runs == 0)
+        return pc_fail(PC_EINVAL, "pc_microcanonical_ci: no runs");
+    if (!d_sum1 || !d_sum2 || !d_beta || !d_out)
+        return pc_fail(PC_EINVAL, "pc_microcanonical_ci: NULL buffer");
+    const uint64_t S = (uint64_t)num_edges + 1u;
+    micro_ci_kernel<<<(uint32_t)((S + 255) / 256), 256, 0, as_stream(stream)>>>(
+        S, num_runs, d_sum1, d_sum2, t_lo, t_hi, d_beta, d_out);
+    PC_LAUNCH_CHECK("micro_ci_kernel");
     return PC_OK;
 }
 

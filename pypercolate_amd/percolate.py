@@ -235,7 +235,8 @@ def _spanning_stats(k, runs, alpha):
 
 class _MicrocanonicalStats(object):
     """Per-step microcanonical statistics over R runs, computed on the GPU
-    from the exact sums (columns: max, spanning, moments 0..4)."""
+    from the exact sums, confidence bounds included (pc_microcanonical_ci;
+    columns: max, spanning, moments 0..4)."""
 
     def __init__(self, ga, runs, alpha, spanning_cluster, perms):
         from . import _engine as E
@@ -246,21 +247,27 @@ class _MicrocanonicalStats(object):
         if self.M:
             eng, res = _run_batch(ga, perms)
             s1, s2 = eng.microcanonical_sums(res)
-            mean, std = E.micro_moments(s1, s2, self.M, runs)
-            mean = mean.cpu().numpy()
-            std = std.cpu().numpy()
-            k_span = (s1[1, :, 0].cpu().numpy().view(np.uint64)).astype(np.float64)
-        else:
-            init = np.array([1.0, 0.0] + [N - 1.0 for N in [self.N] * 5])
-            mean = init[:, None] * np.ones((1, S))
-            std = np.zeros((7, S))
-            k_span = np.zeros(S)
+            out = E.micro_ci(s1, s2, self.M, runs, alpha).cpu().numpy()
+            mom = [6, 9, 12, 15, 18]
+            self.max_mean = out[0]
+            self.max_ci = np.stack([out[1], out[2]], axis=-1)
+            self.mom_mean = out[mom]
+            self.mom_ci = np.stack([out[[m + 1 for m in mom]],
+                                    out[[m + 2 for m in mom]]], axis=-1)
+            if spanning_cluster:
+                self.span_mean = out[3]
+                self.span_ci = np.stack([out[4], out[5]], axis=-1)
+            return
+        # a graph without edges: the n = 0 state only
+        init = np.array([1.0, 0.0] + [N - 1.0 for N in [self.N] * 5])
+        mean = init[:, None] * np.ones((1, S))
+        std = np.zeros((7, S))
         self.max_mean = mean[0]
         self.max_ci = _t_interval(mean[0], std[0], runs, alpha)
         self.mom_mean = mean[2:7]
         self.mom_ci = _t_interval(mean[2:7], std[2:7], runs, alpha)
         if spanning_cluster:
-            self.span_mean, self.span_ci = _spanning_stats(k_span, runs, alpha)
+            self.span_mean, self.span_ci = _spanning_stats(np.zeros(S), runs, alpha)
 
     def step(self, n, ret):
         ret['n'] = n

@@ -299,3 +299,31 @@ def test_bond_canonical_averages_batch(gpu, L, spanning):
         else:
             np.testing.assert_allclose(res[key], ref[key], rtol=1e-11,
                                        atol=1e-13, err_msg=key)
+
+
+@pytest.mark.parametrize('length,runs,spanning', [(16, 40, True), (8, 1, True),
+                                                  (12, 7, False), (32, 300, True)])
+def test_device_ci_equals_host_epilogue(gpu, length, runs, spanning):
+    """pc_microcanonical_ci (the confidence bounds on the device) equals the
+    host epilogue (scipy t interval where std > 0, beta quantiles of the
+    spanning counts) applied to the same exact means and stds, bit for bit."""
+    from pypercolate_amd import _engine as E, _graph
+    from pypercolate_amd.percolate import _spanning_stats, _t_interval
+    eng = E.BondEngine(_graph.square_lattice(length))
+    res = eng.sweep(eng.device_permutations(5, runs))
+    s1, s2 = eng.microcanonical_sums(res)
+    alpha = 2 * 0.15865525393145707
+    out = E.micro_ci(s1, s2, eng.M, runs, alpha).cpu().numpy()
+    mean, std = [t.cpu().numpy() for t in E.micro_moments(s1, s2, eng.M, runs)]
+    ci_max = _t_interval(mean[0], std[0], runs, alpha)
+    assert np.array_equal(out[0], mean[0])
+    assert np.array_equal(out[1], ci_max[:, 0]) and np.array_equal(out[2], ci_max[:, 1])
+    ci_mom = _t_interval(mean[2:7], std[2:7], runs, alpha)
+    for j in range(5):
+        assert np.array_equal(out[6 + 3 * j], mean[2 + j])
+        assert np.array_equal(out[7 + 3 * j], ci_mom[j, :, 0])
+        assert np.array_equal(out[8 + 3 * j], ci_mom[j, :, 1])
+    k = s1[1, :, 0].cpu().numpy().view(np.uint64).astype(np.float64)
+    span_mean, span_ci = _spanning_stats(k, runs, alpha)
+    assert np.array_equal(out[3], span_mean)
+    assert np.array_equal(out[4], span_ci[:, 0]) and np.array_equal(out[5], span_ci[:, 1])
