@@ -18,8 +18,11 @@ from . import _graph
 
 
 def _ndarray_dtype(fields):
-    """NumPy structured dtype from (name, type) pairs (hpc.py:22-31)."""
-    return [(np.str_(key), values) for key, values in fields]
+    """NumPy structured dtype from (name, type) pairs (hpc.py:22-31).  The
+    names are plain str: they compare equal to the reference's np.str_
+    names, and numpy 2 would otherwise keep np.str_ objects in the dtype,
+    whose repr makes .npy files unreadable without pickle."""
+    return [(str(key), values) for key, values in fields]
 
 
 def microcanonical_statistics_dtype(spanning_cluster=True):

@@ -1,0 +1,41 @@
+"""The study driver on the GPU: tasks + reduce equal one batch over all the
+seeds, and a second run resumes from the task checkpoints."""
+import numpy as np
+import pytest
+
+from pypercolate_amd import hpc, percolate, study
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(a):
+    return [n for n in a.dtype.names if n not in ('number_of_runs',)]
+
+
+def test_study_tasks_reduce_and_resume(gpu, tmp_path, monkeypatch):
+    ps = np.linspace(0.4, 0.6, 11)
+    ckpt = str(tmp_path / 'ckpt')
+    out = str(tmp_path / 'study.npz')
+    res = study.run_study(dimensions=(4, 6), runs=40, tasks=4, ps=ps,
+                          checkpoint_dir=ckpt, output=out)
+    seeds = study.study_seeds((4, 6), 40)
+    for L in (4, 6):
+        pg = percolate.percolation_graph(percolate.spanning_2d_grid(L))
+        whole = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds[L], **pg)
+        fin = hpc.finalize_canonical_averages(
+            number_of_nodes=pg['num_nodes'], ps=ps, canonical_averages=whole,
+            alpha=study.ALPHA)
+        assert (res[L]['number_of_runs'] == 40).all()
+        for name in _fields(fin):
+            np.testing.assert_allclose(res[L][name], fin[name], rtol=1e-10,
+                                       atol=1e-14, err_msg=name)
+    # resume: every task is on disk, nothing is recomputed
+    def boom(*a, **k):
+        raise AssertionError('recomputed a finished task')
+    monkeypatch.setattr(hpc, 'bond_canonical_averages_batch', boom)
+    again = study.run_study(dimensions=(4, 6), runs=40, tasks=4, ps=ps,
+                            checkpoint_dir=ckpt)
+    back = study.read_results(out)
+    for L in (4, 6):
+        np.testing.assert_array_equal(again[L], res[L])
+        np.testing.assert_array_equal(back[L], res[L])
