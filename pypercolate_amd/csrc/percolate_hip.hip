@@ -667,7 +667,8 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 atomicMax(&claims[su], mine);
                 atomicMax(&claims[sv], mine);
             }
-            const bool own_u = claims[su] == mine, own_v = claims[sv] == mine;
+            const uint32_t held_u = claims[su], held_v = claims[sv];
+            const bool own_u = held_u == mine, own_v = held_v == mine;
             bool go = cand && own_u && own_v;
             // star: the pending candidates that touch one hub root (the root
             // the first blocked candidate is blocked on, usually the largest
@@ -689,7 +690,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 // held by a member touches a root the star absorbs before
                 // its step: the star stops before it (the slot holder is
                 // the earliest claimant, lane tag + 63 - claims[slot])
-                const uint32_t hu = tag + 63u - claims[su], hv = tag + 63u - claims[sv];
+                const uint32_t hu = tag + 63u - held_u, hv = tag + 63u - held_v;
                 const bool late = cand && !go && !((star >> lane) & 1u) &&
                                   (((star >> (hu & 63u)) & 1u) || ((star >> (hv & 63u)) & 1u));
                 const uint64_t lm = __ballot(late);
@@ -708,8 +709,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 st.decode(at_u ? cu : cv, sh, fh);
                 const uint32_t first = (uint32_t)__builtin_ctzll(star);
                 const uint32_t s_hub = read_lane(sh, first), f_hub = read_lane(fh, first);
-                const uint32_t mine_sx = in_star ? sx : 0u;
-                const uint32_t incl = wave_scan_add(mine_sx);
+                const uint32_t incl = wave_scan_add(in_star ? sx : 0u);
                 const uint32_t total = read_lane(incl, 63u);
                 const uint64_t m1 = __ballot(in_star && (fx & 1u));
                 const uint64_t m2 = __ballot(in_star && (fx & 2u));
@@ -2089,7 +2089,7 @@ static inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 
 // parallel merge rounds per batch before the in-order remainder
 #ifndef PC_MERGE_ROUNDS
-#define PC_MERGE_ROUNDS 2
+#define PC_MERGE_ROUNDS 4
 #endif
 static uint32_t g_merge_rounds = PC_MERGE_ROUNDS;
 
