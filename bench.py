@@ -60,6 +60,9 @@ def parse_args(argv=None):
     ap.add_argument('--bitexact', action='store_true',
                     help='also time the pipeline with bit-exact device '
                          'RandomState permutations')
+    ap.add_argument('--per-run', action='store_true',
+                    help='also time the per-run (hpc) flavour of the canonical '
+                         'averages at L = 128: 10^4 runs, 1000 p')
     ap.add_argument('--batch', action='store_true',
                     help='time the finite-size batch (L = 16..512, 2^26/L^2 '
                          'runs each, one HIP graph) instead of L = 128')
@@ -236,6 +239,83 @@ def canonical_comparison(eng, sums, runs, num_p, reps=5):
                            'speedup_vs_plain_fp64_mfma':
                                out['forms'][best]['speedup_vs_plain_fp64_mfma']}
     return out
+
+
+def per_run_canonical(length, runs, num_p, seed, warmup=1, reps=5):
+    """The per-run (hpc) flavour of the canonical averages at L = 128:
+    hpc.bond_canonical_averages_batch over `runs` device-permutation runs
+    (Philox `seed`), num_p values of p in [0.45, 0.55]: per chunk of runs,
+    sweep -> per-run observables over the union of the windows -> FP64
+    matrix-core contraction of every run's 7 columns -> mean and M2 over
+    the chunk's runs -> Chan merge.  Reports the whole call (wall clock,
+    host merges included) and, on its own, the contraction of one chunk
+    (HIP events, median of `reps`) against the FP64 matrix peak, with the
+    algorithmic flops 2 * sum_p |window_p| * 7 * runs of the chunk.
+    ``graph_ingest_ms`` is the host share of each call that reads the
+    networkx graph into index arrays."""
+    import torch
+    from pypercolate_amd import hpc, percolate
+    from pypercolate_amd import _engine as E
+    from pypercolate_amd._engine import Weights
+    pg = percolate.percolation_graph(percolate.spanning_2d_grid(length),
+                                     spanning_cluster=True)
+    ps = np.linspace(0.45, 0.55, num_p)
+    # host share of every call: the networkx graph -> index arrays pass
+    t = time.perf_counter()
+    hpc._graph_arrays(pg['perc_graph'], True, pg['auxiliary_node_attributes'],
+                      pg['auxiliary_edge_attributes'], pg['spanning_sides'])
+    ingest_ms = (time.perf_counter() - t) * 1e3
+
+    def call():
+        return hpc.bond_canonical_averages_batch(ps=ps, num_runs=runs,
+                                                 philox_seed=seed, **pg)
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    avg = call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    M = pg['num_edges']
+    wt = E.binomial_weights(M, ps)
+    g_lo = int(wt.host_start.min())
+    g_len = int((wt.host_start + wt.host_length).max()) - g_lo
+    shifted = Weights(M, ps, wt.start - g_lo, wt.length, wt.kmax, wt.w)
+    shifted.host_start = wt.host_start - g_lo
+    shifted.host_length = wt.host_length
+    chunk = min(runs, 1024)
+    x = torch.rand((g_len, 7 * chunk), dtype=torch.float64, device='cuda')
+    plan = E.ContractPlan(x, shifted, 'fp64_mfma')
+    plan.run()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(reps):
+        e0.record()
+        plan.run()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    cms = sorted(times)[len(times) // 2]
+    flops = 2.0 * float(wt.host_length.sum()) * x.shape[1]
+    ok = bool(avg['number_of_runs'][0] == runs and
+              np.all((avg['percolation_probability_mean'] >= 0) &
+                     (avg['percolation_probability_mean'] <= 1)))
+    return {
+        'workload': ('hpc.bond_canonical_averages_batch, L={} (N={}, M={}), {} '
+                     'runs from Philox seed {}, {} p in [0.45, 0.55], 7 '
+                     'columns per run').format(length, pg['num_nodes'], M,
+                                               runs, seed, num_p),
+        'ms': ms, 'graph_ingest_ms': ingest_ms,
+        'runs_per_s': runs / (ms / 1e3),
+        'bond_additions_per_s': float(runs) * M / (ms / 1e3),
+        'contraction_chunk': {
+            'runs': chunk, 'columns': int(x.shape[1]), 'steps': g_len,
+            'ms': cms, 'algorithmic_flops': flops,
+            'tflops': flops / cms / 1e9, 'peak_tflops': 78.6,
+            'frac': flops / cms / 1e9 / 78.6},
+        'check': ok,
+    }
 
 
 def batch_bench(args):
@@ -530,6 +610,9 @@ def main(argv=None):
     # would mix into the kernel statistics of the Philox pipeline.
     if args.bitexact:
         out['bitexact_mode'] = bitexact_mode(eng, R, args.steps, start, stop)
+    if args.per_run:
+        out['per_run_canonical'] = per_run_canonical(args.length, R, 1000,
+                                                     args.seed)
     if args.canonical_length > 0:
         info, big, sums, runs = large_lattice(args.canonical_length,
                                               args.canonical_runs, seed=7)

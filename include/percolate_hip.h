@@ -217,7 +217,8 @@ int pc_window_traces(uint32_t num_nodes, uint32_t num_edges, uint32_t num_runs,
  * (mean, M2) over runs of per-run canonical statistics
  * d_q[p * (R * cols) + r * cols + c]: the number_of_runs / *_mean / *_m2 of
  * reduce(bond_reduce, map(bond_initialize_canonical_averages, ...))
- * (percolate/hpc.py:561-702), two-pass in a fixed order.
+ * (percolate/hpc.py:561-702), two-pass in a fixed order.  cols <= 8
+ * (7 observables with spanning, 6 without); more is PC_EINVAL.
  */
 int pc_run_moments(const double *d_q, uint32_t num_p, uint32_t num_runs,
                    uint32_t cols, double *d_mean, double *d_m2, void *stream);

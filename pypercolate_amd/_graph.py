@@ -136,12 +136,13 @@ def from_percolation_graph(perc_graph, spanning_cluster=True,
     """GraphArrays of the keyword dict percolate.percolation_graph() returns
     (percolate/percolate.py:31-100), or of the same keywords passed to the
     percolate.hpc functions."""
-    nodes = list(perc_graph.nodes())
+    # comprehensions, not list(): list() asks the networkx views for their
+    # length first, which sums the degrees of every node
+    nodes = [node for node in perc_graph.nodes()]
     index = {node: i for i, node in enumerate(nodes)}
-    edge_list = list(perc_graph.edges())
-    edges = np.array(
-        [(index[u], index[v]) for (u, v) in edge_list], dtype=np.int32,
-    ).reshape(-1, 2)
+    edge_list = [edge for edge in perc_graph.edges()]
+    edges = np.fromiter((index[node] for edge in edge_list for node in edge),
+                        dtype=np.int32, count=2 * len(edge_list)).reshape(-1, 2)
     node_sides = None
     span_first = False
     if spanning_cluster:

@@ -1335,6 +1335,16 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(
 // same k; register r of the output holds D[4 * (lane / 16) + r][lane % 16]
 // for the FP32 accumulators and D[lane / 16 + 4 r][lane % 16] for FP64.
 // ---------------------------------------------------------------------------
+// Row tile of workgroup b out of T: workgroups are dealt to the 8 XCDs in
+// turn, so give each XCD a contiguous run of tiles; neighbouring p-values
+// read the same rows of X and then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
+{
+    const uint32_t x = b % 8u, idx = b / 8u;
+    const uint32_t base = T / 8u, extra = T % 8u;
+    return x * base + min(x, extra) + idx;
+}
+
 typedef double v4f64 __attribute__((ext_vector_type(4)));
 typedef float v4f32 __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
@@ -1475,6 +1485,138 @@ __global__ __launch_bounds__(64) void contract_mfma_kernel(
     }
 }
 
+// Wide contraction on the FP64 matrix cores, for many columns (the per-run
+// canonical statistics of a whole chunk of runs at once): a GEMM of the
+// banded weights with X.  Workgroup of 4 wavefronts on a tile of 64 p x 64
+// columns; wave w takes the p rows 16w..16w+15 and all 64 columns, four
+// v_mfma_f64_16x16x4f64 accumulators sharing the weight operand.  The steps
+// of the union of the tile's 64 windows advance 16 at a time: the 16 x 64
+// block of X is staged through LDS (double-buffered, one barrier per step;
+// rows padded to 68 doubles so the four lane groups of a read fall on
+// different banks), each lane's 4 weights are consecutive steps.  Tiles are
+// dealt XCD-contiguous in column-major order, so the p tiles that read the
+// same columns of X share an L2.  Partial sums of K-slices (blockIdx.y) go
+// to `dst` + slice * P * C and contract_reduce_kernel adds them in order.
+#define WIDE_TILE 64
+#define WIDE_KSTEP 16
+#define WIDE_LDS_STRIDE 68
+
+__global__ __launch_bounds__(256) void contract_wide_f64_kernel(
+    const double *__restrict__ x, int64_t S, const double *__restrict__ w,
+    TileCtx t, uint32_t tiles_p, uint32_t slices, double *__restrict__ dst)
+{
+    __shared__ double xs[2][WIDE_KSTEP * WIDE_LDS_STRIDE];
+    __shared__ int64_t range[2];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t p0 = (tile % tiles_p) * WIDE_TILE;
+    const uint32_t c0 = (tile / tiles_p) * WIDE_TILE;
+    const uint32_t slice = blockIdx.y;
+    const uint32_t tid = threadIdx.x, wave = tid / 64u, lane = tid % 64u;
+    const uint32_t i = lane % 16u, q = lane / 16u;
+    if (wave == 0) {
+        // union of the windows of the tile's p
+        int64_t lo = INT64_MAX, hi = 0;
+        if (p0 + lane < t.P) {
+            lo = t.start[p0 + lane];
+            hi = lo + t.len[p0 + lane];
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, (int64_t)__shfl_xor((long long)lo, off, 64));
+            hi = max(hi, (int64_t)__shfl_xor((long long)hi, off, 64));
+        }
+        if (lane == 0) {
+            range[0] = lo;
+            range[1] = hi;
+        }
+    }
+    __syncthreads();
+    const int64_t lo = range[0], hi = range[1];
+    const int64_t steps = hi > lo ? (hi - lo + WIDE_KSTEP - 1) / WIDE_KSTEP : 0;
+    const int64_t per = (steps + slices - 1) / slices;
+    const int64_t g_begin = lo + (int64_t)slice * per * WIDE_KSTEP;
+    const int64_t g_end = min(hi, g_begin + per * WIDE_KSTEP);
+
+    const uint32_t p = p0 + wave * 16u + i;
+    const bool prow = p < t.P;
+    const int64_t wstart = prow ? t.start[p] : 0, wlen = prow ? t.len[p] : 0;
+    const double *wrow = w + (int64_t)(prow ? p : 0) * t.kmax;
+    // X loader: thread tid fills row tid / 16, columns 4 (tid % 16) .. + 3
+    const uint32_t lr = tid / 16u, lc = 4u * (tid % 16u);
+
+    v4f64 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        acc[j] = v4f64{0.0, 0.0, 0.0, 0.0};
+    double a_cur[4], a_nxt[4], b_nxt[4];
+    // weights of steps g + 4q .. g + 4q + 3 for row p (zero outside its
+    // window); X rows past the slice end are zero, so a partial last step
+    // adds nothing that belongs to the next slice
+    auto load_a = [&](int64_t g, double *a) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t k = g + 4 * (int64_t)q + e - wstart;
+            a[e] = (k >= 0 && k < wlen) ? wrow[k] : 0.0;
+        }
+    };
+    auto load_b = [&](int64_t g, double *b) {
+        const int64_t row = g + lr;
+        const bool ok = row < g_end && row < S;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t c = c0 + lc + (uint32_t)e;
+            b[e] = (ok && c < t.C) ? x[row * t.x_stride + c] : 0.0;
+        }
+    };
+    auto store_b = [&](int buf, const double *b) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            xs[buf][lr * WIDE_LDS_STRIDE + lc + e] = b[e];
+    };
+    if (g_begin < g_end) {
+        load_b(g_begin, b_nxt);
+        load_a(g_begin, a_cur);
+        store_b(0, b_nxt);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t g = g_begin; g < g_end; g += WIDE_KSTEP) {
+            const bool more = g + WIDE_KSTEP < g_end;
+            if (more) {
+                load_b(g + WIDE_KSTEP, b_nxt);
+                load_a(g + WIDE_KSTEP, a_nxt);
+            }
+            const double *xb = xs[buf];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                // step g + 4q + kk: weight a_cur[kk], X row 4q + kk
+                const double *xr = xb + (4u * q + (uint32_t)kk) * WIDE_LDS_STRIDE + i;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        a_cur[kk], xr[16 * j], acc[j], 0, 0, 0);
+            }
+            if (more) {
+                store_b(buf ^ 1, b_nxt);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    a_cur[e] = a_nxt[e];
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    // register r of accumulator j holds D[q + 4r][i] of column block j
+    double *out = dst + (int64_t)slice * t.P * t.C;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t row = p0 + wave * 16u + q + 4u * (uint32_t)r;
+            const uint32_t col = c0 + 16u * (uint32_t)j + i;
+            if (row < t.P && col < t.C)
+                out[(int64_t)row * t.C + col] = acc[j][r];
+        }
+}
+
 __global__ __launch_bounds__(256) void contract_reduce_kernel(
     const double *__restrict__ partial, uint32_t slices, int64_t n,
     double *__restrict__ out)
@@ -1582,16 +1724,6 @@ __device__ __forceinline__ void walk_init(BinomWalk &b, const FusedCtx &f,
     b.kend = min(k1, st + ln);
     b.anchor = b.ka < b.kend ? binom_pmf_outline((double)b.ka, b.n, b.p) : 0.0;
     b.w = 0.0;
-}
-
-// Row tile of workgroup b out of T: workgroups are dealt to the 8 XCDs in
-// turn, so give each XCD a contiguous run of tiles; neighbouring p-values
-// read the same rows of X and then share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t T)
-{
-    const uint32_t x = b % 8u, idx = b / 8u;
-    const uint32_t base = T / 8u, extra = T % 8u;
-    return x * base + min(x, extra) + idx;
 }
 
 // Weight at k (k advances by one per call) from the weight at k - 1 and
@@ -1897,26 +2029,67 @@ __global__ __launch_bounds__(64) void window_traces_kernel(
 
 // Mean and sum of squared deviations over runs of q[p][r * cols + c]
 // (two passes, fixed order): the (n, mean, M2) of percolate/hpc.py:561-702.
+// One workgroup per p reads that p's contiguous row of R * cols values:
+// thread t takes runs t, t + 256, ... (cols consecutive values each), and
+// the 256 partial sums of a column meet in a fixed LDS tree, so the result
+// does not depend on scheduling.
+#define RUN_MOMENTS_COLS 8
+
+__device__ __forceinline__ double block_sum_256(double v, double *red)
+{
+    // fixed-order tree: wave-local butterfly, then the 4 wave totals in order
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    const uint32_t wave = threadIdx.x / 64;
+    __syncthreads();
+    if (threadIdx.x % 64 == 0)
+        red[wave] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ __launch_bounds__(256) void run_moments_kernel(
     const double *__restrict__ q, uint32_t P, uint32_t R, uint32_t cols,
     double *__restrict__ mean, double *__restrict__ m2)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (uint64_t)P * cols)
-        return;
-    const uint64_t p = i / cols, c = i % cols;
-    const double *row = q + p * (uint64_t)R * cols + c;
-    double s = 0.0;
-    for (uint32_t r = 0; r < R; ++r)
-        s += row[(uint64_t)r * cols];
-    const double mu = s / (double)R;
-    double d2 = 0.0;
-    for (uint32_t r = 0; r < R; ++r) {
-        const double d = row[(uint64_t)r * cols] - mu;
-        d2 = fma(d, d, d2);
+    __shared__ double red[4];
+    const uint32_t p = blockIdx.x;
+    const double *row = q + (uint64_t)p * R * cols;
+    double s[RUN_MOMENTS_COLS];
+#pragma unroll
+    for (int c = 0; c < RUN_MOMENTS_COLS; ++c)
+        s[c] = 0.0;
+    for (uint32_t r = threadIdx.x; r < R; r += 256) {
+#pragma unroll
+        for (int c = 0; c < RUN_MOMENTS_COLS; ++c)
+            if (c < (int)cols)
+                s[c] += row[(uint64_t)r * cols + c];
     }
-    mean[i] = mu;
-    m2[i] = d2;
+    double mu[RUN_MOMENTS_COLS];
+#pragma unroll
+    for (int c = 0; c < RUN_MOMENTS_COLS; ++c)
+        mu[c] = c < (int)cols ? block_sum_256(s[c], red) / (double)R : 0.0;
+#pragma unroll
+    for (int c = 0; c < RUN_MOMENTS_COLS; ++c)
+        s[c] = 0.0;
+    for (uint32_t r = threadIdx.x; r < R; r += 256) {
+#pragma unroll
+        for (int c = 0; c < RUN_MOMENTS_COLS; ++c)
+            if (c < (int)cols) {
+                const double d = row[(uint64_t)r * cols + c] - mu[c];
+                s[c] = fma(d, d, s[c]);
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < RUN_MOMENTS_COLS; ++c) {
+        if (c < (int)cols) {
+            const double d2 = block_sum_256(s[c], red);
+            if (threadIdx.x == 0) {
+                mean[(uint64_t)p * cols + c] = mu[c];
+                m2[(uint64_t)p * cols + c] = d2;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2426,16 +2599,46 @@ static uint32_t contract_slices(uint32_t num_p, uint32_t cols)
     return (uint32_t)sl;
 }
 
+// The FP64 matrix-core contraction takes the wide (64 x 64 tile) kernel from
+// this many columns on.
+#define WIDE_MIN_COLS 64u
+
+static bool contract_wide(uint32_t cols, int32_t precision)
+{
+    return precision == PC_PREC_FP64_MFMA && cols >= WIDE_MIN_COLS;
+}
+
+static uint32_t wide_tiles_p(uint32_t num_p) { return (num_p + WIDE_TILE - 1u) / WIDE_TILE; }
+static uint32_t wide_tiles(uint32_t num_p, uint32_t cols)
+{
+    return wide_tiles_p(num_p) * ((cols + WIDE_TILE - 1u) / WIDE_TILE);
+}
+
+// K-slices of the wide kernel: none from four workgroups per CU on (1024
+// tiles), otherwise enough to reach about that many, at most 16.
+static uint32_t wide_slices(uint32_t num_p, uint32_t cols)
+{
+    const uint64_t tiles = wide_tiles(num_p, cols);
+    if (tiles >= 1024u)
+        return 1u;
+    uint64_t sl = (1024u + tiles - 1u) / (tiles ? tiles : 1u);
+    return (uint32_t)(sl > PC_CONTRACT_SLICES ? PC_CONTRACT_SLICES : sl);
+}
+
 extern "C" int pc_contract_workspace_size(uint32_t num_p, uint32_t cols,
                                           int32_t precision, uint64_t *bytes)
 {
     g_last_error[0] = 0;
     if (!bytes)
         return pc_fail(PC_EINVAL, "pc_contract_workspace_size: bytes is NULL");
-    *bytes = precision == PC_PREC_FP64
-                 ? 0u
-                 : (uint64_t)contract_slices(num_p, cols) * num_p * cols *
-                       sizeof(double);
+    if (precision == PC_PREC_FP64)
+        *bytes = 0u;
+    else if (contract_wide(cols, precision)) {
+        const uint32_t sl = wide_slices(num_p, cols);
+        *bytes = sl > 1u ? (uint64_t)sl * num_p * cols * sizeof(double) : 0u;
+    } else
+        *bytes = (uint64_t)contract_slices(num_p, cols) * num_p * cols *
+                 sizeof(double);
     return PC_OK;
 }
 
@@ -2473,10 +2676,26 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
     }
     uint64_t need = 0;
     pc_contract_workspace_size(num_p, cols, precision, &need);
-    if (!d_workspace || workspace_bytes < need)
+    if (need > 0u && (!d_workspace || workspace_bytes < need))
         return pc_fail(PC_ENOMEM, "pc_canonical_contract: workspace needs %llu bytes",
                        (unsigned long long)need);
     const TileCtx t{num_p, cols, x_stride, kmax, d_start, d_len};
+    if (contract_wide(cols, precision)) {
+        const uint32_t sl = wide_slices(num_p, cols);
+        const dim3 grid(wide_tiles(num_p, cols), sl);
+        double *dst = sl > 1u ? (double *)d_workspace : d_out;
+        contract_wide_f64_kernel<<<grid, 256, 0, s>>>(
+            (const double *)d_x, num_steps, (const double *)d_w, t,
+            wide_tiles_p(num_p), sl, dst);
+        PC_LAUNCH_CHECK("contract_wide_f64_kernel");
+        if (sl > 1u) {
+            const int64_t n = (int64_t)num_p * cols;
+            contract_reduce_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(
+                dst, sl, n, d_out);
+            PC_LAUNCH_CHECK("contract_reduce_kernel");
+        }
+        return PC_OK;
+    }
     const uint32_t slices = contract_slices(num_p, cols);
     const dim3 grid((num_p + 15u) / 16u, (cols + 15u) / 16u, slices);
     double *partial = (double *)d_workspace;
@@ -2678,8 +2897,9 @@ extern "C" int pc_run_moments(const double *d_q, uint32_t num_p,
         return PC_OK;
     if (num_runs == 0 || !d_q || !d_mean || !d_m2)
         return pc_fail(PC_EINVAL, "pc_run_moments: bad arguments");
-    const uint64_t n = (uint64_t)num_p * cols;
-    run_moments_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+    if (cols > RUN_MOMENTS_COLS)
+        return pc_fail(PC_EINVAL, "pc_run_moments: at most 8 columns");
+    run_moments_kernel<<<num_p, 256, 0, as_stream(stream)>>>(
         d_q, num_p, num_runs, cols, d_mean, d_m2);
     PC_LAUNCH_CHECK("run_moments_kernel");
     return PC_OK;

@@ -389,7 +389,7 @@ def finalize_canonical_averages(
 def bond_canonical_averages_batch(
     perc_graph, num_nodes, num_edges, ps, seeds=None, num_runs=None,
     philox_seed=0, spanning_cluster=True, auxiliary_node_attributes=None,
-    auxiliary_edge_attributes=None, spanning_sides=None, run_chunk=1024,
+    auxiliary_edge_attributes=None, spanning_sides=None, run_chunk=None,
     **kwargs
 ):
     """Canonical averages over many runs in batched GPU launches: the result
@@ -405,7 +405,9 @@ def bond_canonical_averages_batch(
     over the union of the convolution windows, one FP64 matrix-core
     contraction of all runs' columns with the binomial weights, and the
     per-(p, field) mean and M2 over the chunk's runs; chunks merge with the
-    Chan formula.  Returns an array of canonical_averages_dtype (one row per
+    Chan formula.  ``run_chunk`` (default: three sweep residencies of the
+    GPU, 3072 runs at L = 128) is capped so that one chunk's columns take
+    at most 1 GiB.  Returns an array of canonical_averages_dtype (one row per
     p), ready for finalize_canonical_averages.
     """
     import torch
@@ -437,6 +439,9 @@ def bond_canonical_averages_batch(
     shifted = Weights(M, ps, wt.start - g_lo, wt.length, wt.kmax, wt.w)
     shifted.host_start = wt.host_start - g_lo
     shifted.host_length = wt.host_length
+    if run_chunk is None:
+        cus = torch.cuda.get_device_properties(eng.device).multi_processor_count
+        run_chunk = 3 * max(eng.occupancy(), 1) * cus
     chunk = max(1, min(int(run_chunk), (1 << 30) // (g_len * cols * 8)))
 
     n_acc = None
