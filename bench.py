@@ -252,7 +252,8 @@ def per_run_canonical(length, runs, num_p, seed, warmup=1, reps=5):
     (HIP events, median of `reps`) against the FP64 matrix peak, with the
     algorithmic flops 2 * sum_p |window_p| * 7 * runs of the chunk.
     ``graph_ingest_ms`` is the host share of each call that reads the
-    networkx graph into index arrays."""
+    networkx graph into index arrays; ``ms_with_graph_arrays`` is the same
+    call given the closed-form arrays of the lattice instead."""
     import torch
     from pypercolate_amd import hpc, percolate
     from pypercolate_amd import _engine as E
@@ -266,16 +267,23 @@ def per_run_canonical(length, runs, num_p, seed, warmup=1, reps=5):
                       pg['auxiliary_edge_attributes'], pg['spanning_sides'])
     ingest_ms = (time.perf_counter() - t) * 1e3
 
-    def call():
+    from pypercolate_amd import _graph
+    ga = _graph.square_lattice(length)
+
+    def call(**extra):
         return hpc.bond_canonical_averages_batch(ps=ps, num_runs=runs,
-                                                 philox_seed=seed, **pg)
+                                                 philox_seed=seed, **extra, **pg)
+
+    def wall(**extra):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = call(**extra)
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t) * 1e3
     for _ in range(warmup):
         call()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    avg = call()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t) * 1e3
+    avg, ms = wall()
+    avg_ga, ms_ga = wall(graph_arrays=ga)
     M = pg['num_edges']
     wt = E.binomial_weights(M, ps)
     g_lo = int(wt.host_start.min())
@@ -299,6 +307,7 @@ def per_run_canonical(length, runs, num_p, seed, warmup=1, reps=5):
     cms = sorted(times)[len(times) // 2]
     flops = 2.0 * float(wt.host_length.sum()) * x.shape[1]
     ok = bool(avg['number_of_runs'][0] == runs and
+              all(np.array_equal(avg[k], avg_ga[k]) for k in avg.dtype.names) and
               np.all((avg['percolation_probability_mean'] >= 0) &
                      (avg['percolation_probability_mean'] <= 1)))
     return {
@@ -307,7 +316,9 @@ def per_run_canonical(length, runs, num_p, seed, warmup=1, reps=5):
                      'columns per run').format(length, pg['num_nodes'], M,
                                                runs, seed, num_p),
         'ms': ms, 'graph_ingest_ms': ingest_ms,
+        'ms_with_graph_arrays': ms_ga,
         'runs_per_s': runs / (ms / 1e3),
+        'runs_per_s_with_graph_arrays': runs / (ms_ga / 1e3),
         'bond_additions_per_s': float(runs) * M / (ms / 1e3),
         'contraction_chunk': {
             'runs': chunk, 'columns': int(x.shape[1]), 'steps': g_len,

@@ -1487,29 +1487,35 @@ __global__ __launch_bounds__(64) void contract_mfma_kernel(
 
 // Wide contraction on the FP64 matrix cores, for many columns (the per-run
 // canonical statistics of a whole chunk of runs at once): a GEMM of the
-// banded weights with X.  Workgroup of 4 wavefronts on a tile of 64 p x 64
-// columns; wave w takes the p rows 16w..16w+15 and all 64 columns, four
+// banded weights with X.  Workgroup of 4 wavefronts on a tile of 64 p x TC
+// columns; wave w takes the p rows 16w..16w+15 and all TC columns, TC / 16
 // v_mfma_f64_16x16x4f64 accumulators sharing the weight operand.  The steps
-// of the union of the tile's 64 windows advance 16 at a time: the 16 x 64
+// of the union of the tile's 64 windows advance 16 at a time: the 16 x TC
 // block of X is staged through LDS (double-buffered, one barrier per step;
-// rows padded to 68 doubles so the four lane groups of a read fall on
+// rows padded by 4 doubles so the four lane groups of a read fall on
 // different banks), each lane's 4 weights are consecutive steps.  Tiles are
 // dealt XCD-contiguous in column-major order, so the p tiles that read the
 // same columns of X share an L2.  Partial sums of K-slices (blockIdx.y) go
 // to `dst` + slice * P * C and contract_reduce_kernel adds them in order.
 #define WIDE_TILE 64
+#ifndef WIDE_KSTEP
 #define WIDE_KSTEP 16
-#define WIDE_LDS_STRIDE 68
+#endif
 
+// TC columns per tile (64 or 128): TC / 16 accumulators per wave; LDS rows
+// of TC + 4 doubles; WIDE_KSTEP (16 or 32) steps per barrier
+template <int TC>
 __global__ __launch_bounds__(256) void contract_wide_f64_kernel(
     const double *__restrict__ x, int64_t S, const double *__restrict__ w,
     TileCtx t, uint32_t tiles_p, uint32_t slices, double *__restrict__ dst)
 {
-    __shared__ double xs[2][WIDE_KSTEP * WIDE_LDS_STRIDE];
+    constexpr int KS = WIDE_KSTEP, H = KS / 16;
+    constexpr int STRIDE = TC + 4, NJ = TC / 16, E = TC / 16;
+    __shared__ double xs[2][KS * STRIDE];
     __shared__ int64_t range[2];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t p0 = (tile % tiles_p) * WIDE_TILE;
-    const uint32_t c0 = (tile / tiles_p) * WIDE_TILE;
+    const uint32_t c0 = (tile / tiles_p) * TC;
     const uint32_t slice = blockIdx.y;
     const uint32_t tid = threadIdx.x, wave = tid / 64u, lane = tid % 64u;
     const uint32_t i = lane % 16u, q = lane / 16u;
@@ -1531,46 +1537,54 @@ __global__ __launch_bounds__(256) void contract_wide_f64_kernel(
     }
     __syncthreads();
     const int64_t lo = range[0], hi = range[1];
-    const int64_t steps = hi > lo ? (hi - lo + WIDE_KSTEP - 1) / WIDE_KSTEP : 0;
+    const int64_t steps = hi > lo ? (hi - lo + KS - 1) / KS : 0;
     const int64_t per = (steps + slices - 1) / slices;
-    const int64_t g_begin = lo + (int64_t)slice * per * WIDE_KSTEP;
-    const int64_t g_end = min(hi, g_begin + per * WIDE_KSTEP);
+    const int64_t g_begin = lo + (int64_t)slice * per * KS;
+    const int64_t g_end = min(hi, g_begin + per * KS);
 
     const uint32_t p = p0 + wave * 16u + i;
     const bool prow = p < t.P;
     const int64_t wstart = prow ? t.start[p] : 0, wlen = prow ? t.len[p] : 0;
     const double *wrow = w + (int64_t)(prow ? p : 0) * t.kmax;
-    // X loader: thread tid fills row tid / 16, columns 4 (tid % 16) .. + 3
-    const uint32_t lr = tid / 16u, lc = 4u * (tid % 16u);
+    // X loader: thread tid fills rows tid / 16 + 16 h, columns
+    // E (tid % 16) .. + E-1
+    const uint32_t lr = tid / 16u, lc = (uint32_t)E * (tid % 16u);
 
-    v4f64 acc[4];
+    v4f64 acc[NJ];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NJ; ++j)
         acc[j] = v4f64{0.0, 0.0, 0.0, 0.0};
-    double a_cur[4], a_nxt[4], b_nxt[4];
-    // weights of steps g + 4q .. g + 4q + 3 for row p (zero outside its
+    double a_cur[4 * H], a_nxt[4 * H], b_nxt[E * H];
+    // weights of steps g + 16h + 4q .. + 3 for row p (zero outside its
     // window); X rows past the slice end are zero, so a partial last step
     // adds nothing that belongs to the next slice
     auto load_a = [&](int64_t g, double *a) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t k = g + 4 * (int64_t)q + e - wstart;
-            a[e] = (k >= 0 && k < wlen) ? wrow[k] : 0.0;
-        }
+        for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t k = g + 16 * h + 4 * (int64_t)q + e - wstart;
+                a[4 * h + e] = (k >= 0 && k < wlen) ? wrow[k] : 0.0;
+            }
     };
     auto load_b = [&](int64_t g, double *b) {
-        const int64_t row = g + lr;
-        const bool ok = row < g_end && row < S;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t c = c0 + lc + (uint32_t)e;
-            b[e] = (ok && c < t.C) ? x[row * t.x_stride + c] : 0.0;
+        for (int h = 0; h < H; ++h) {
+            const int64_t row = g + 16 * h + lr;
+            const bool ok = row < g_end && row < S;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t c = c0 + lc + (uint32_t)e;
+                b[E * h + e] = (ok && c < t.C) ? x[row * t.x_stride + c] : 0.0;
+            }
         }
     };
     auto store_b = [&](int buf, const double *b) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-            xs[buf][lr * WIDE_LDS_STRIDE + lc + e] = b[e];
+        for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                xs[buf][(16 * h + lr) * STRIDE + lc + e] = b[E * h + e];
     };
     if (g_begin < g_end) {
         load_b(g_begin, b_nxt);
@@ -1578,26 +1592,30 @@ __global__ __launch_bounds__(256) void contract_wide_f64_kernel(
         store_b(0, b_nxt);
         __syncthreads();
         int buf = 0;
-        for (int64_t g = g_begin; g < g_end; g += WIDE_KSTEP) {
-            const bool more = g + WIDE_KSTEP < g_end;
+        for (int64_t g = g_begin; g < g_end; g += KS) {
+            const bool more = g + KS < g_end;
             if (more) {
-                load_b(g + WIDE_KSTEP, b_nxt);
-                load_a(g + WIDE_KSTEP, a_nxt);
+                load_b(g + KS, b_nxt);
+                load_a(g + KS, a_nxt);
             }
             const double *xb = xs[buf];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                // step g + 4q + kk: weight a_cur[kk], X row 4q + kk
-                const double *xr = xb + (4u * q + (uint32_t)kk) * WIDE_LDS_STRIDE + i;
+            for (int h = 0; h < H; ++h)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                        a_cur[kk], xr[16 * j], acc[j], 0, 0, 0);
-            }
+                for (int kk = 0; kk < 4; ++kk) {
+                    // step g + 16h + 4q + kk: weight a_cur[4h + kk], X row
+                    // 16h + 4q + kk
+                    const double *xr =
+                        xb + (16u * h + 4u * q + (uint32_t)kk) * STRIDE + i;
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j)
+                        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                            a_cur[4 * h + kk], xr[16 * j], acc[j], 0, 0, 0);
+                }
             if (more) {
                 store_b(buf ^ 1, b_nxt);
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
+                for (int e = 0; e < 4 * H; ++e)
                     a_cur[e] = a_nxt[e];
             }
             __syncthreads();
@@ -1607,7 +1625,7 @@ __global__ __launch_bounds__(256) void contract_wide_f64_kernel(
     // register r of accumulator j holds D[q + 4r][i] of column block j
     double *out = dst + (int64_t)slice * t.P * t.C;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const uint32_t row = p0 + wave * 16u + q + 4u * (uint32_t)r;
@@ -2599,9 +2617,12 @@ static uint32_t contract_slices(uint32_t num_p, uint32_t cols)
     return (uint32_t)sl;
 }
 
-// The FP64 matrix-core contraction takes the wide (64 x 64 tile) kernel from
-// this many columns on.
+// The FP64 matrix-core contraction takes the wide kernel from this many
+// columns on, with tiles of 64 p x WIDE_TC columns.
 #define WIDE_MIN_COLS 64u
+#ifndef WIDE_TC
+#define WIDE_TC 64
+#endif
 
 static bool contract_wide(uint32_t cols, int32_t precision)
 {
@@ -2611,7 +2632,7 @@ static bool contract_wide(uint32_t cols, int32_t precision)
 static uint32_t wide_tiles_p(uint32_t num_p) { return (num_p + WIDE_TILE - 1u) / WIDE_TILE; }
 static uint32_t wide_tiles(uint32_t num_p, uint32_t cols)
 {
-    return wide_tiles_p(num_p) * ((cols + WIDE_TILE - 1u) / WIDE_TILE);
+    return wide_tiles_p(num_p) * ((cols + WIDE_TC - 1u) / WIDE_TC);
 }
 
 // K-slices of the wide kernel: none from four workgroups per CU on (1024
@@ -2684,7 +2705,7 @@ extern "C" int pc_canonical_contract(uint32_t num_steps, uint32_t cols,
         const uint32_t sl = wide_slices(num_p, cols);
         const dim3 grid(wide_tiles(num_p, cols), sl);
         double *dst = sl > 1u ? (double *)d_workspace : d_out;
-        contract_wide_f64_kernel<<<grid, 256, 0, s>>>(
+        contract_wide_f64_kernel<WIDE_TC><<<grid, 256, 0, s>>>(
             (const double *)d_x, num_steps, (const double *)d_w, t,
             wide_tiles_p(num_p), sl, dst);
         PC_LAUNCH_CHECK("contract_wide_f64_kernel");

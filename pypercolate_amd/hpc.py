@@ -390,7 +390,7 @@ def bond_canonical_averages_batch(
     perc_graph, num_nodes, num_edges, ps, seeds=None, num_runs=None,
     philox_seed=0, spanning_cluster=True, auxiliary_node_attributes=None,
     auxiliary_edge_attributes=None, spanning_sides=None, run_chunk=None,
-    **kwargs
+    graph_arrays=None, **kwargs
 ):
     """Canonical averages over many runs in batched GPU launches: the result
     of the reference's jug study for one system size,
@@ -407,7 +407,9 @@ def bond_canonical_averages_batch(
     per-(p, field) mean and M2 over the chunk's runs; chunks merge with the
     Chan formula.  ``run_chunk`` (default: three sweep residencies of the
     GPU, 3072 runs at L = 128) is capped so that one chunk's columns take
-    at most 1 GiB.  Returns an array of canonical_averages_dtype (one row per
+    at most 1 GiB.  ``graph_arrays`` (a ``_graph.GraphArrays`` of the same
+    graph, e.g. ``_graph.square_lattice(L)``) skips reading the networkx
+    graph, which is most of the host time of a call at L = 128.  Returns an array of canonical_averages_dtype (one row per
     p), ready for finalize_canonical_averages.
     """
     import torch
@@ -415,8 +417,15 @@ def bond_canonical_averages_batch(
     from ._engine import BondEngine, ContractPlan, Weights
 
     _check_sides(spanning_cluster, spanning_sides)
-    ga = _graph_arrays(perc_graph, spanning_cluster, auxiliary_node_attributes,
-                       auxiliary_edge_attributes, spanning_sides)
+    if graph_arrays is not None:
+        ga = graph_arrays
+        if ga.spanning != bool(spanning_cluster):
+            raise ValueError('graph_arrays and spanning_cluster disagree on '
+                             'spanning sides')
+    else:
+        ga = _graph_arrays(perc_graph, spanning_cluster,
+                           auxiliary_node_attributes,
+                           auxiliary_edge_attributes, spanning_sides)
     if ga.num_nodes != num_nodes or ga.num_edges != num_edges:
         raise ValueError('num_nodes / num_edges do not match perc_graph')
     if ga.num_edges == 0:

@@ -26,6 +26,7 @@ from functools import reduce
 import numpy as np
 import scipy.stats
 
+from . import _graph
 from . import hpc
 from . import percolate
 
@@ -64,13 +65,17 @@ def run_study(dimensions=(8, 16, 32), runs=10000, tasks=100,
         graph = percolate.spanning_2d_grid(length=L)
         pg = percolate.percolation_graph(graph=graph,
                                          spanning_cluster=spanning_cluster)
+        # the closed form of this graph's index arrays (same nodes, edges,
+        # edge order and sides), so no task reads the networkx graph again
+        ga = _graph.square_lattice(L, spanning_cluster=spanning_cluster)
         reduced = []
         for t, chunk in enumerate(np.split(seeds[L], tasks)):
             path = _task_path(checkpoint_dir, L, t) if checkpoint_dir else None
             if path and os.path.exists(path):
                 reduced.append(np.load(path, allow_pickle=False))
                 continue
-            avg = hpc.bond_canonical_averages_batch(ps=ps, seeds=chunk, **pg)
+            avg = hpc.bond_canonical_averages_batch(ps=ps, seeds=chunk,
+                                                    graph_arrays=ga, **pg)
             if path:
                 tmp = path + '.tmp.npy'
                 np.save(tmp, avg, allow_pickle=False)

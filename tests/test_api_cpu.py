@@ -114,3 +114,18 @@ def test_percolation_graph_keys():
     assert len(pg['auxiliary_edge_attributes']) == 8
     pg2 = percolate.percolation_graph(nx.grid_2d_graph(3, 3), spanning_cluster=False)
     assert pg2['num_edges'] == 12 and 'spanning_sides' not in pg2
+
+
+def test_batch_graph_arrays_must_match_spanning():
+    """graph_arrays without sides for a spanning-cluster call (or a graph of
+    another size) is a ValueError, raised before any GPU work."""
+    from pypercolate_amd import _graph
+    pg = percolate.percolation_graph(percolate.spanning_2d_grid(4),
+                                     spanning_cluster=True)
+    with pytest.raises(ValueError):
+        hpc.bond_canonical_averages_batch(
+            ps=[0.5], num_runs=1,
+            graph_arrays=_graph.square_lattice(4, spanning_cluster=False), **pg)
+    with pytest.raises(ValueError):
+        hpc.bond_canonical_averages_batch(
+            ps=[0.5], num_runs=1, graph_arrays=_graph.square_lattice(5), **pg)

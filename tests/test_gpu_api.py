@@ -327,3 +327,20 @@ def test_device_ci_equals_host_epilogue(gpu, length, runs, spanning):
     span_mean, span_ci = _spanning_stats(k, runs, alpha)
     assert np.array_equal(out[3], span_mean)
     assert np.array_equal(out[4], span_ci[:, 0]) and np.array_equal(out[5], span_ci[:, 1])
+
+
+@pytest.mark.parametrize('spanning', [True, False])
+def test_batch_graph_arrays_closed_form(gpu, spanning):
+    """graph_arrays=_graph.square_lattice(L) gives the same averages as
+    reading the networkx graph (same node order, edge order and sides)."""
+    from pypercolate_amd import _graph
+    pg = percolate.percolation_graph(percolate.spanning_2d_grid(12),
+                                     spanning_cluster=spanning)
+    ps = np.linspace(0.35, 0.65, 7)
+    seeds = list(range(20))
+    a = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds, **pg)
+    b = hpc.bond_canonical_averages_batch(
+        ps=ps, seeds=seeds,
+        graph_arrays=_graph.square_lattice(12, spanning_cluster=spanning), **pg)
+    for key in a.dtype.names:
+        assert np.array_equal(a[key], b[key]), key
