@@ -125,12 +125,13 @@ class BondEngine(object):
         return int(v.value)
 
     def fuse_chunk_states(self):
-        """Whether the sweep should write the chunk states itself.  Measured
-        on MI355X: with the union-find state in LDS (one run per SIMD, every
-        cycle on the critical path) the extra wave reductions slow the sweep
-        by ~25 %, more than the separate pass costs when it runs on another
-        stream; with the state in HBM the sweep hides them and the separate
-        pass (one wavefront walking each long run) is the slower choice."""
+        """Whether the sweep should write the chunk states itself (its finder
+        wave folds the records one batch behind the resolver).  Measured on
+        MI355X: with the union-find state in LDS the fold still lengthens the
+        sweep by ~12 % (L = 128: 7.7 -> 8.7 ms for 10^4 runs), more than the
+        separate pass (pc_chunk_scan) costs on another stream; with the state
+        in HBM the sweep hides it and the separate pass (one wavefront
+        walking each long run) is the slower choice."""
         lds = self.N <= 16384 and self.state_mode != nat.PC_STATE_HBM
         return not lds
 

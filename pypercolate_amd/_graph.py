@@ -157,7 +157,10 @@ def from_percolation_graph(perc_graph, spanning_cluster=True,
 
 
 def square_lattice(length, spanning_cluster=True):
-    """Closed form of percolation_graph(spanning_2d_grid(length)).
+    """Closed form of percolation_graph(spanning_2d_grid(length)) with
+    spanning_cluster=True (with spanning_cluster=False: the L x L grid
+    alone, without spanning sides; percolation_graph then keeps the whole
+    grid, auxiliary rows included, which this is not).
 
     Nodes (i, j), i = 1..L, j = 0..L-1 of the grid get index (i-1)*L + j; the
     edges come in networkx's order for grid_2d_graph subgraphs: row by row,

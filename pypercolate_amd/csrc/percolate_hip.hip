@@ -221,6 +221,11 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
 #define PERM_ROUNDS 8
 #define PERM_BLOCK 256
 #define PERM_ITEMS 16
+// elements per thread mapped together (more measured slower: the kernel is
+// bound by the table reads, not by their latency)
+#ifndef PERM_ILP
+#define PERM_ILP 1
+#endif
 #define PERM_TAG 0x7065726du
 
 __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
@@ -248,23 +253,50 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
 
     int32_t *out = perm + (int64_t)blockIdx.y * stride;
     const uint32_t base = blockIdx.x * (PERM_BLOCK * PERM_ITEMS);
-#pragma unroll 4
-    for (int k = 0; k < PERM_ITEMS; ++k) {
-        const uint32_t i = base + (uint32_t)k * PERM_BLOCK + threadIdx.x;
-        if (i >= M)
-            break;
-        uint32_t y = i;
-        do {
-            uint32_t L = y >> h, R = y & mask;
+    // PERM_ILP elements per thread at a time, their round chains interleaved
+    for (int k0 = 0; k0 < PERM_ITEMS; k0 += PERM_ILP) {
+        uint32_t y[PERM_ILP];
+        bool need[PERM_ILP];
+        bool any = false;
 #pragma unroll
-            for (int j = 0; j < PERM_ROUNDS; ++j) {
-                const uint32_t t = L ^ table[((uint32_t)j << h) | R];
-                L = R;
-                R = t;
+        for (int e = 0; e < PERM_ILP; ++e) {
+            y[e] = base + (uint32_t)(k0 + e) * PERM_BLOCK + threadIdx.x;
+            need[e] = y[e] < M;
+            any = any || need[e];
+        }
+        // cycle walking: an element whose image lies outside [0, M) is
+        // mapped again (its chain is computed alongside the others')
+        while (any) {
+            uint32_t L[PERM_ILP], R[PERM_ILP];
+#pragma unroll
+            for (int e = 0; e < PERM_ILP; ++e) {
+                L[e] = y[e] >> h;
+                R[e] = y[e] & mask;
             }
-            y = (L << h) | R;
-        } while (y >= M);
-        out[i] = (int32_t)y;
+#pragma unroll
+            for (int j = 0; j < PERM_ROUNDS; ++j)
+#pragma unroll
+                for (int e = 0; e < PERM_ILP; ++e) {
+                    const uint32_t t = L[e] ^ table[((uint32_t)j << h) | R[e]];
+                    L[e] = R[e];
+                    R[e] = t;
+                }
+            any = false;
+#pragma unroll
+            for (int e = 0; e < PERM_ILP; ++e) {
+                if (need[e]) {
+                    y[e] = (L[e] << h) | R[e];
+                    need[e] = y[e] >= M;
+                }
+                any = any || need[e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < PERM_ILP; ++e) {
+            const uint32_t i = base + (uint32_t)(k0 + e) * PERM_BLOCK + threadIdx.x;
+            if (i < M)
+                out[i] = (int32_t)y[e];
+        }
     }
 }
 
@@ -589,12 +621,20 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 // (the chunk state of chunk_scan) is written as well: the batch's records
 // are in registers at its end, so their deltas are reduced across the wave
 // there and no pass over the records is needed.
-template <class St, typename RecT, bool CHUNKS>
+// SPLIT (two wavefronts per run): wave 1, the finder, runs step 1 for batch
+// b + 1 while wave 0, the resolver, runs steps 2 and 3 for batch b; one
+// barrier per batch, the roots handed over through LDS.  A find that runs
+// beside the merges of batch b may stop at a root linked meanwhile, which is
+// still an ancestor: the resolver's find continues from it, usually with a
+// single read.  With chunk_out, the finder also folds the records of batch
+// b - 1 into the running state, so the reductions stay off the resolver.
+template <class St, typename RecT, bool CHUNKS, bool SPLIT>
 __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           const int2 *__restrict__ edges,
                                           const uint8_t *__restrict__ sides,
                                           bool span_first, uint32_t rounds,
                                           uint32_t *__restrict__ claims,
+                                          uint32_t *__restrict__ handoff,
                                           const int32_t *__restrict__ perm,
                                           RecT *__restrict__ rec_out,
                                           uint32_t *__restrict__ span_out,
@@ -607,12 +647,18 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     // HBM-state graphs (N > 16384) need 128-bit S3 and S4
     constexpr bool WIDE = std::is_same<St, GlobalState>::value;
     using MT = typename Wide<WIDE>::T;
-    const uint32_t lane = threadIdx.x;
+    constexpr uint32_t THREADS = SPLIT ? 128u : 64u;
+    const uint32_t lane = threadIdx.x % 64u;
+    // SPLIT: wave 0 resolves batch b while wave 1 finds the roots of batch
+    // b + 1 (a find that runs beside the merges of batch b may stop at a
+    // root linked meanwhile; that is still an ancestor, and the resolver's
+    // find continues from it)
+    const bool resolver = !SPLIT || threadIdx.x < 64u;
     uint64_t cs_mx = 1, cs_merges = 0, cs_s2 = N;
     MT cs_s3 = N, cs_s4 = N;
-    for (uint32_t i = lane; i < N; i += 64u)
+    for (uint32_t i = threadIdx.x; i < N; i += THREADS)
         st.set(i, st.root(1u, sides ? (uint32_t)sides[i] : 0u));
-    for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
+    for (uint32_t i = threadIdx.x; i < CLAIM_SLOTS; i += THREADS)
         claims[i] = 0u;
     __syncthreads();
 
@@ -621,6 +667,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 
     // software pipeline: the permutation two batches ahead and the edge
     // endpoints one batch ahead are in flight while a batch is resolved
+    // (SPLIT: while the finder wave works on the batch before)
     int32_t e_ahead = (64u + lane < M) ? perm[64u + lane] : 0;
     int2 uv_next = make_int2(0, 0);
     if (lane < M)
@@ -633,22 +680,76 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     // (the node may have been linked since): that find reads afresh.
     uint32_t hint_u = 0xFFFFFFFFu, hint_v = 0xFFFFFFFFu;
 
+    // SPLIT finder: roots of the batch at b0 into handoff slot (b0 / 64) & 1
+    // (u roots in words 0..63, v roots in 64..127 of the slot)
+    auto find_batch = [&](uint32_t b0) {
+        const int2 uv = uv_next;
+        uv_next = (b0 + 64u + lane < M) ? edges[e_ahead] : make_int2(0, 0);
+        e_ahead = (b0 + 128u + lane < M) ? perm[b0 + 128u + lane] : 0;
+        uint32_t x = (uint32_t)uv.x, y = (uint32_t)uv.y, cx, cy;
+        if (b0 + lane < M)
+            find2(st, x, cx, y, cy);
+        uint32_t *slot = handoff + ((b0 / 64u) & 1u) * 128u;
+        slot[lane] = x;
+        slot[64u + lane] = y;
+    };
+    if (SPLIT) {
+        if (!resolver)
+            find_batch(0u);
+        __syncthreads();
+    }
+
+    // SPLIT with chunk states: the resolver hands each batch's records to
+    // the finder (two slots of 64 records after the root slots), which folds
+    // them into the running state one batch later, off the resolver's chain
+    RecT *rec_slots = (RecT *)(handoff + 256u);
+    auto fold_batch = [&](RecT rec) {
+        const Deltas<WIDE> d(rec);
+        cs_mx = umax64(cs_mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
+        cs_merges += read_lane(wave_scan_add(d.cnt), 63u);
+        cs_s2 += read_lane(wave_scan_add(d.d2), 63u);
+        cs_s3 += read_lane(wave_scan_add(d.d3), 63u);
+        cs_s4 += read_lane(wave_scan_add(d.d4), 63u);
+    };
+
     for (uint32_t base = 0; base < M; base += 64u) {
+        if (SPLIT && !resolver) {
+            if (CHUNKS) {
+                // state at the start of this batch: fold the records the
+                // resolver finished in the previous iteration
+                if (base > 0u)
+                    fold_batch(rec_slots[(((base / 64u) - 1u) & 1u) * 64u + lane]);
+                if (lane < PC_STATE_WORDS)
+                    chunk_out[(size_t)(base / 64u) * PC_STATE_WORDS + lane] =
+                        state_word(lane, cs_mx, cs_merges, cs_s2, cs_s3, cs_s4);
+            }
+            if (base + 64u < M)
+                find_batch(base + 64u);
+            __syncthreads();
+            continue;
+        }
         const uint32_t n = base + lane;
         const bool active = n < M;
-        if (CHUNKS && lane < PC_STATE_WORDS)
+        if (!SPLIT && CHUNKS && lane < PC_STATE_WORDS)
             chunk_out[(size_t)(base / 64u) * PC_STATE_WORDS + lane] =
                 state_word(lane, cs_mx, cs_merges, cs_s2, cs_s3, cs_s4);
-        const int2 uv = uv_next;
-        uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
-        e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
-
-        uint32_t ru = st.is_root(hint_u) ? (uint32_t)uv.x : hint_u;
-        uint32_t rv = st.is_root(hint_v) ? (uint32_t)uv.y : hint_v;
-        uint32_t cu, cv;
-        find2(st, ru, cu, rv, cv);
-        hint_u = st.get((uint32_t)uv_next.x);
-        hint_v = st.get((uint32_t)uv_next.y);
+        uint32_t ru, rv, cu, cv;
+        if (SPLIT) {
+            // the finder's roots, followed past the previous batch's merges
+            const uint32_t *slot = handoff + ((base / 64u) & 1u) * 128u;
+            ru = slot[lane];
+            rv = slot[64u + lane];
+            find2(st, ru, cu, rv, cv);
+        } else {
+            const int2 uv = uv_next;
+            uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
+            e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
+            ru = st.is_root(hint_u) ? (uint32_t)uv.x : hint_u;
+            rv = st.is_root(hint_v) ? (uint32_t)uv.y : hint_v;
+            find2(st, ru, cu, rv, cv);
+            hint_u = st.get((uint32_t)uv_next.x);
+            hint_v = st.get((uint32_t)uv_next.y);
+        }
         bool cand = active && ru != rv;
         uint64_t pending = __ballot(cand);
         RecT rec = 0;
@@ -800,22 +901,30 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         }
         if (active)
             rec_out[n] = rec;
-        if (CHUNKS) {
-            const Deltas<WIDE> d(rec);
-            cs_mx = umax64(cs_mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
-            cs_merges += read_lane(wave_scan_add(d.cnt), 63u);
-            cs_s2 += read_lane(wave_scan_add(d.d2), 63u);
-            cs_s3 += read_lane(wave_scan_add(d.d3), 63u);
-            cs_s4 += read_lane(wave_scan_add(d.d4), 63u);
+        if (SPLIT && CHUNKS)
+            rec_slots[((base / 64u) & 1u) * 64u + lane] = rec;
+        else if (CHUNKS)
+            fold_batch(rec);
+        if (!SPLIT)
+            __syncthreads();
+        else {
+            st.sync();
+            __syncthreads();
         }
-        __syncthreads();
     }
-    if (lane == 0)
+    if (threadIdx.x == 0)
         *span_out = span_at;
 }
 
-template <typename RecT, bool CHUNKS>
-__global__ __launch_bounds__(64) void sweep_lds_kernel(
+// Words of LDS before the 16-bit codes: the claim table, then (SPLIT) the
+// handoff: two slots of 128 roots, then two slots of 64 records (up to 64
+// bits each).
+#define SWEEP_HANDOFF_WORDS 512u
+template <bool SPLIT>
+constexpr uint32_t sweep_lds_words() { return CLAIM_SLOTS + (SPLIT ? SWEEP_HANDOFF_WORDS : 0u); }
+
+template <typename RecT, bool CHUNKS, bool SPLIT>
+__global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
     uint32_t N, uint32_t M, const int2 *__restrict__ edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
@@ -823,19 +932,20 @@ __global__ __launch_bounds__(64) void sweep_lds_kernel(
     uint32_t *__restrict__ span_step, uint32_t rounds,
     uint64_t *__restrict__ chunk_state)
 {
-    // claim table first (word aligned), then the 16-bit codes
+    // claim table first (word aligned), the handoff slots, then the codes
     extern __shared__ uint32_t lds_words[];
     const uint32_t r = blockIdx.x;
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
-    const LdsState st{(uint16_t *)(lds_words + CLAIM_SLOTS), N};
-    sweep_run<LdsState, RecT, CHUNKS>(
+    const LdsState st{(uint16_t *)(lds_words + sweep_lds_words<SPLIT>()), N};
+    sweep_run<LdsState, RecT, CHUNKS, SPLIT>(
         st, N, M, edges, sides, span_first != 0, rounds, lds_words,
-        perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
-        span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
+        lds_words + CLAIM_SLOTS, perm + (int64_t)r * perm_stride,
+        records + (int64_t)r * rec_stride, span_step + r,
+        CHUNKS ? chunk_state + r * cstride : nullptr);
 }
 
-template <typename RecT, bool CHUNKS>
-__global__ __launch_bounds__(64) void sweep_global_kernel(
+template <typename RecT, bool CHUNKS, bool SPLIT>
+__global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
     uint32_t N, uint32_t M, uint32_t R, const int2 *__restrict__ edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
@@ -844,11 +954,12 @@ __global__ __launch_bounds__(64) void sweep_global_kernel(
     uint32_t rounds, uint64_t *__restrict__ chunk_state)
 {
     __shared__ uint32_t claims[CLAIM_SLOTS];
+    __shared__ uint32_t handoff[SPLIT ? SWEEP_HANDOFF_WORDS : 1u];
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
-        sweep_run<GlobalState, RecT, CHUNKS>(
-            st, N, M, edges, sides, span_first != 0, rounds, claims,
+        sweep_run<GlobalState, RecT, CHUNKS, SPLIT>(
+            st, N, M, edges, sides, span_first != 0, rounds, claims, handoff,
             perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
             span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
         __syncthreads();
@@ -2284,6 +2395,18 @@ static inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 #endif
 static uint32_t g_merge_rounds = PC_MERGE_ROUNDS;
 
+// Two wavefronts per run (a finder and a resolver, see sweep_run) or one.
+#ifndef PC_SWEEP_SPLIT
+#define PC_SWEEP_SPLIT 1
+#endif
+
+template <bool SPLIT>
+static size_t sweep_lds_bytes(uint32_t num_nodes)
+{
+    return sweep_lds_words<SPLIT>() * sizeof(uint32_t) +
+           (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
+}
+
 extern "C" int pc_abi_version(void) { return 1; }
 
 extern "C" const char *pc_last_error(void) { return g_last_error; }
@@ -2370,16 +2493,17 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     const int2 *edges = (const int2 *)d_edges;
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
+    constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
+    constexpr uint32_t threads = SPLIT ? 128u : 64u;
     if (use_lds) {
-        const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
-                           (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
+        const size_t lds = sweep_lds_bytes<SPLIT>(num_nodes);
         if (d_chunk_state)
-            sweep_lds_kernel<uint32_t, true><<<num_runs, 64, lds, s>>>(
+            sweep_lds_kernel<uint32_t, true, SPLIT><<<num_runs, threads, lds, s>>>(
                 num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
                 d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
                 d_span_step, rounds, d_chunk_state);
         else
-            sweep_lds_kernel<uint32_t, false><<<num_runs, 64, lds, s>>>(
+            sweep_lds_kernel<uint32_t, false, SPLIT><<<num_runs, threads, lds, s>>>(
                 num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
                 d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
                 d_span_step, rounds, nullptr);
@@ -2399,7 +2523,7 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     if (slots > kGlobalSlots)
         slots = kGlobalSlots;
 #define PC_SWEEP_GLOBAL(RT, CH)                                                \
-    sweep_global_kernel<RT, CH><<<(uint32_t)slots, 64, 0, s>>>(                \
+    sweep_global_kernel<RT, CH, SPLIT><<<(uint32_t)slots, threads, 0, s>>>(    \
         num_nodes, num_edges, num_runs, edges, d_node_sides,                   \
         span_on_first_merge, d_perm, perm_stride, (RT *)d_records, rec_stride, \
         d_span_step, (uint32_t *)d_workspace, rounds, d_chunk_state)
@@ -2858,13 +2982,14 @@ extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
         (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes)) {
         if (num_nodes > kLdsMaxNodes)
             return pc_fail(PC_EINVAL, "pc_sweep_occupancy: too many nodes for LDS");
-        const size_t lds = CLAIM_SLOTS * sizeof(uint32_t) +
-                           (((size_t)num_nodes * sizeof(uint16_t) + 3u) & ~(size_t)3u);
+        constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_lds_kernel<uint32_t, false>, 64, lds));
+            &blocks, sweep_lds_kernel<uint32_t, false, SPLIT>, SPLIT ? 128 : 64,
+            sweep_lds_bytes<SPLIT>(num_nodes)));
     } else {
+        constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_global_kernel<uint32_t, false>, 64, 0));
+            &blocks, sweep_global_kernel<uint32_t, false, SPLIT>, SPLIT ? 128 : 64, 0));
     }
     *blocks_per_cu = blocks;
     return PC_OK;

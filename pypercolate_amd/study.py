@@ -65,9 +65,8 @@ def run_study(dimensions=(8, 16, 32), runs=10000, tasks=100,
         graph = percolate.spanning_2d_grid(length=L)
         pg = percolate.percolation_graph(graph=graph,
                                          spanning_cluster=spanning_cluster)
-        # the closed form of this graph's index arrays (same nodes, edges,
-        # edge order and sides), so no task reads the networkx graph again
-        ga = _graph.square_lattice(L, spanning_cluster=spanning_cluster)
+        # the graph's index arrays, read once per size rather than per task
+        ga = _graph.from_percolation_graph(**pg)
         reduced = []
         for t, chunk in enumerate(np.split(seeds[L], tasks)):
             path = _task_path(checkpoint_dir, L, t) if checkpoint_dir else None
