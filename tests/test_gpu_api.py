@@ -330,17 +330,21 @@ def test_device_ci_equals_host_epilogue(gpu, length, runs, spanning):
 
 
 @pytest.mark.parametrize('spanning', [True, False])
-def test_batch_graph_arrays_closed_form(gpu, spanning):
-    """graph_arrays=_graph.square_lattice(L) gives the same averages as
-    reading the networkx graph (same node order, edge order and sides)."""
+def test_batch_graph_arrays_prepared(gpu, spanning):
+    """Prepared graph_arrays give the same averages as reading the networkx
+    graph in the call: the closed form _graph.square_lattice(L) for the
+    spanning lattice (same node order, edge order and sides), the arrays
+    read once from the graph without spanning (the whole grid, auxiliary
+    rows included)."""
     from pypercolate_amd import _graph
     pg = percolate.percolation_graph(percolate.spanning_2d_grid(12),
                                      spanning_cluster=spanning)
+    ga = (_graph.square_lattice(12) if spanning
+          else _graph.from_percolation_graph(**pg))
     ps = np.linspace(0.35, 0.65, 7)
     seeds = list(range(20))
     a = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds, **pg)
-    b = hpc.bond_canonical_averages_batch(
-        ps=ps, seeds=seeds,
-        graph_arrays=_graph.square_lattice(12, spanning_cluster=spanning), **pg)
+    b = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds,
+                                          graph_arrays=ga, **pg)
     for key in a.dtype.names:
         assert np.array_equal(a[key], b[key]), key

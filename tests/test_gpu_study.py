@@ -39,3 +39,22 @@ def test_study_tasks_reduce_and_resume(gpu, tmp_path, monkeypatch):
     for L in (4, 6):
         np.testing.assert_array_equal(again[L], res[L])
         np.testing.assert_array_equal(back[L], res[L])
+
+
+def test_study_without_spanning(gpu):
+    """spanning_cluster=False: the whole grid (auxiliary rows included) is
+    the percolation graph; the study equals one batch over all the seeds."""
+    ps = np.linspace(0.4, 0.6, 5)
+    res = study.run_study(dimensions=(5,), runs=20, tasks=2, ps=ps,
+                          spanning_cluster=False)
+    seeds = study.study_seeds((5,), 20)
+    pg = percolate.percolation_graph(percolate.spanning_2d_grid(5),
+                                     spanning_cluster=False)
+    whole = hpc.bond_canonical_averages_batch(ps=ps, seeds=seeds[5], **pg)
+    fin = hpc.finalize_canonical_averages(
+        number_of_nodes=pg['num_nodes'], ps=ps, canonical_averages=whole,
+        alpha=study.ALPHA)
+    assert 'percolation_probability_mean' not in fin.dtype.names
+    for name in _fields(fin):
+        np.testing.assert_allclose(res[5][name], fin[name], rtol=1e-10,
+                                   atol=1e-14, err_msg=name)
