@@ -72,6 +72,17 @@ int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
 #define PC_STATE_HBM 2  /* 32-bit codes in the HBM workspace                 */
 
 /*
+ * Wavefronts per run, OR-ed into state_mode of pc_sweep, pc_sweep_occupancy
+ * and pc_sweep_workspace_size (neither: the library picks, two from 4096
+ * nodes on or with the state in HBM).  Two: a finder wavefront finds the
+ * roots of the next batch of bonds while the resolver merges this one.
+ * Results do not depend on it.
+ */
+#define PC_SWEEP_ONE_WAVE 0x10
+#define PC_SWEEP_TWO_WAVES 0x20
+#define PC_SWEEP_WAVES_MASK 0x30
+
+/*
  * Bit-exact device permutations: d_perm[r * perm_stride + i] =
  * numpy.random.RandomState(d_seeds[r]).permutation(num_edges)[i], the
  * permutation the reference draws per run (percolate/hpc.py:195,206), from
@@ -86,7 +97,7 @@ int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
                             uint32_t state_mode, uint64_t *bytes);
 
 /*
- * The Newman-Ziff sweep of num_runs independent runs, one per wavefront:
+ * The Newman-Ziff sweep of num_runs independent runs, one per workgroup:
  * replaces the per-edge loop of percolate/hpc.py:249-307 (and its twin
  * percolate/percolate.py:298-356).  Reads d_perm[r * perm_stride + n],
  * writes merge records d_records[r * rec_stride + n] (n < num_edges) and
@@ -94,7 +105,7 @@ int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
  * (pc_sweep_workspace_size bytes) per state_mode (PC_STATE_*).
  * merge_rounds is the number of parallel conflict-free merge rounds per
  * batch of 64 bonds before the in-order remainder (0 = library default);
- * results do not depend on it or on state_mode.  With d_chunk_state
+ * results do not depend on it or on state_mode (PC_STATE_* | PC_SWEEP_*).  With d_chunk_state
  * non-NULL the sweep also writes the chunk states of pc_chunk_scan (same
  * layout, d_chunk_state[(r * C + c) * 8 + w]), so that pass is not needed.
  */

@@ -58,7 +58,9 @@ class BondEngine(object):
         self.rec_bytes = self.lib.pc_record_bytes(self.N)
         self.rec_torch = torch.int32 if self.rec_bytes == 4 else torch.int64
         self.rec_numpy = nat.record_dtype(self.N)
-        # sweep knobs (results do not depend on them); set by the autotuner
+        # sweep knobs (results do not depend on them); set by the autotuner:
+        # merge rounds per batch and state placement (PC_STATE_*) OR-ed with
+        # the wavefronts per run (PC_SWEEP_*)
         self.merge_rounds = 0
         self.state_mode = nat.PC_STATE_AUTO
 
@@ -132,7 +134,8 @@ class BondEngine(object):
         separate pass (pc_chunk_scan) costs on another stream; with the state
         in HBM the sweep hides it and the separate pass (one wavefront
         walking each long run) is the slower choice."""
-        lds = self.N <= 16384 and self.state_mode != nat.PC_STATE_HBM
+        placement = self.state_mode & ~nat.PC_SWEEP_WAVES_MASK
+        lds = self.N <= 16384 and placement != nat.PC_STATE_HBM
         return not lds
 
     def sweep(self, d_perm, records=None, span_step=None, workspace=None,

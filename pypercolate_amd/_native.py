@@ -24,6 +24,9 @@ PC_NO_SPAN = 0xFFFFFFFF
 PC_STATE_AUTO = 0
 PC_STATE_LDS = 1
 PC_STATE_HBM = 2
+PC_SWEEP_ONE_WAVE = 0x10
+PC_SWEEP_TWO_WAVES = 0x20
+PC_SWEEP_WAVES_MASK = 0x30
 PC_CHUNK = 64
 PC_STATE_WORDS = 8
 PC_COLUMNS = 7

@@ -4,6 +4,7 @@ For a graph, ``tune_sweep`` times the sweep on the device over a sample of
 runs for every candidate
   * number of parallel conflict-free merge rounds per batch of 64 bonds,
   * union-find state placement (LDS for N <= 16384, or the HBM workspace),
+  * one or two wavefronts per run (a finder beside the resolver),
 and keeps the fastest.  The sample is one residency's worth of runs (the
 HIP occupancy calculator's workgroups per CU times the number of CUs, at
 most about 2^28 bonds of buffers), so the timing sees the concurrency of a
@@ -81,7 +82,9 @@ def _time_sweep(engine, perm, records, span, reps=3):
 def tune_sweep(engine, sample_runs=None, rounds=ROUND_CANDIDATES, modes=None,
                use_cache=True, seed=0):
     """Pick (merge_rounds, state_mode) for ``engine`` (a BondEngine), set
-    them on it and return a dict with the choice and the timings."""
+    them on it and return a dict with the choice and the timings.
+    state_mode candidates are PC_STATE_LDS / PC_STATE_HBM, each with one
+    and with two wavefronts per run (PC_SWEEP_*)."""
     torch = engine.torch
     device = torch.cuda.get_device_properties(engine.device)
     key = '{}|{}|{}'.format(device.gcnArchName, device.multi_processor_count,
@@ -93,9 +96,11 @@ def tune_sweep(engine, sample_runs=None, rounds=ROUND_CANDIDATES, modes=None,
         engine.state_mode = int(choice['state_mode'])
         return choice
     if modes is None:
-        modes = [nat.PC_STATE_HBM]
+        places = [nat.PC_STATE_HBM]
         if engine.N <= 16384:
-            modes = [nat.PC_STATE_LDS, nat.PC_STATE_HBM]
+            places = [nat.PC_STATE_LDS, nat.PC_STATE_HBM]
+        modes = [p | w for p in places
+                 for w in (nat.PC_SWEEP_ONE_WAVE, nat.PC_SWEEP_TWO_WAVES)]
     cus = device.multi_processor_count
     timings = {}
     best = None

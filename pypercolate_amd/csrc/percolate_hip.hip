@@ -2395,10 +2395,22 @@ static inline hipStream_t as_stream(void *s) { return (hipStream_t)s; }
 #endif
 static uint32_t g_merge_rounds = PC_MERGE_ROUNDS;
 
-// Two wavefronts per run (a finder and a resolver, see sweep_run) or one.
-#ifndef PC_SWEEP_SPLIT
-#define PC_SWEEP_SPLIT 1
-#endif
+// Two wavefronts per run (a finder and a resolver, see sweep_run) or one:
+// unless the caller forces one (PC_SWEEP_ONE_WAVE / PC_SWEEP_TWO_WAVES in
+// state_mode), two for HBM state and for LDS state from kSplitMinNodes nodes
+// on.  Measured on MI355X: below that the second wave halves the runs a CU
+// holds (L = 16: 4.8e10 additions/s with one wave, 3.8e10 with two), above
+// it the finder wins (L = 64: 7.0e10 -> 7.4e10; L = 128: 3.3e10 -> 4.1e10).
+static const uint32_t kSplitMinNodes = 4096u;
+
+static bool sweep_split(uint32_t num_nodes, bool use_lds, uint32_t waves)
+{
+    if (waves == PC_SWEEP_ONE_WAVE)
+        return false;
+    if (waves == PC_SWEEP_TWO_WAVES)
+        return true;
+    return !use_lds || num_nodes >= kSplitMinNodes;
+}
 
 template <bool SPLIT>
 static size_t sweep_lds_bytes(uint32_t num_nodes)
@@ -2449,6 +2461,7 @@ extern "C" int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
     g_last_error[0] = 0;
     if (!bytes)
         return pc_fail(PC_EINVAL, "pc_sweep_workspace_size: bytes is NULL");
+    state_mode &= ~PC_SWEEP_WAVES_MASK;
     if (state_mode == PC_STATE_LDS ||
         (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes)) {
         *bytes = 0;
@@ -2471,8 +2484,10 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     const uint32_t rounds = merge_rounds ? merge_rounds : g_merge_rounds;
     if (rounds > 64)
         return pc_fail(PC_EINVAL, "pc_sweep: merge_rounds > 64");
-    if (state_mode > PC_STATE_HBM)
-        return pc_fail(PC_EINVAL, "pc_sweep: unknown state_mode %u", state_mode);
+    const uint32_t waves = state_mode & PC_SWEEP_WAVES_MASK;
+    state_mode &= ~PC_SWEEP_WAVES_MASK;
+    if (state_mode > PC_STATE_HBM || waves == PC_SWEEP_WAVES_MASK)
+        return pc_fail(PC_EINVAL, "pc_sweep: unknown state_mode %u", state_mode | waves);
     if (state_mode == PC_STATE_LDS && num_nodes > kLdsMaxNodes)
         return pc_fail(PC_EINVAL, "pc_sweep: LDS state needs num_nodes <= %u",
                        kLdsMaxNodes);
@@ -2493,20 +2508,24 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     const int2 *edges = (const int2 *)d_edges;
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
-    constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
-    constexpr uint32_t threads = SPLIT ? 128u : 64u;
+    const bool split = sweep_split(num_nodes, use_lds, waves);
+    const uint32_t threads = split ? 128u : 64u;
     if (use_lds) {
-        const size_t lds = sweep_lds_bytes<SPLIT>(num_nodes);
-        if (d_chunk_state)
-            sweep_lds_kernel<uint32_t, true, SPLIT><<<num_runs, threads, lds, s>>>(
-                num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
-                d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
-                d_span_step, rounds, d_chunk_state);
+        const size_t lds = split ? sweep_lds_bytes<true>(num_nodes)
+                                 : sweep_lds_bytes<false>(num_nodes);
+#define PC_SWEEP_LDS(CH, SP, CS)                                                    \
+    sweep_lds_kernel<uint32_t, CH, SP><<<num_runs, threads, lds, s>>>(               \
+        num_nodes, num_edges, edges, d_node_sides, span_on_first_merge, d_perm,     \
+        perm_stride, (uint32_t *)d_records, rec_stride, d_span_step, rounds, CS)
+        if (d_chunk_state && split)
+            PC_SWEEP_LDS(true, true, d_chunk_state);
+        else if (d_chunk_state)
+            PC_SWEEP_LDS(true, false, d_chunk_state);
+        else if (split)
+            PC_SWEEP_LDS(false, true, nullptr);
         else
-            sweep_lds_kernel<uint32_t, false, SPLIT><<<num_runs, threads, lds, s>>>(
-                num_nodes, num_edges, edges, d_node_sides, span_on_first_merge,
-                d_perm, perm_stride, (uint32_t *)d_records, rec_stride,
-                d_span_step, rounds, nullptr);
+            PC_SWEEP_LDS(false, false, nullptr);
+#undef PC_SWEEP_LDS
         PC_LAUNCH_CHECK("sweep_lds_kernel");
         return PC_OK;
     }
@@ -2522,19 +2541,27 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         slots = num_runs;
     if (slots > kGlobalSlots)
         slots = kGlobalSlots;
-#define PC_SWEEP_GLOBAL(RT, CH)                                                \
-    sweep_global_kernel<RT, CH, SPLIT><<<(uint32_t)slots, threads, 0, s>>>(    \
+#define PC_SWEEP_GLOBAL(RT, CH, SP)                                            \
+    sweep_global_kernel<RT, CH, SP><<<(uint32_t)slots, threads, 0, s>>>(       \
         num_nodes, num_edges, num_runs, edges, d_node_sides,                   \
         span_on_first_merge, d_perm, perm_stride, (RT *)d_records, rec_stride, \
         d_span_step, (uint32_t *)d_workspace, rounds, d_chunk_state)
+#define PC_SWEEP_GLOBAL_W(RT, CH)                                              \
+    do {                                                                       \
+        if (split)                                                             \
+            PC_SWEEP_GLOBAL(RT, CH, true);                                     \
+        else                                                                   \
+            PC_SWEEP_GLOBAL(RT, CH, false);                                    \
+    } while (0)
     if (u32rec && d_chunk_state)
-        PC_SWEEP_GLOBAL(uint32_t, true);
+        PC_SWEEP_GLOBAL_W(uint32_t, true);
     else if (u32rec)
-        PC_SWEEP_GLOBAL(uint32_t, false);
+        PC_SWEEP_GLOBAL_W(uint32_t, false);
     else if (d_chunk_state)
-        PC_SWEEP_GLOBAL(uint64_t, true);
+        PC_SWEEP_GLOBAL_W(uint64_t, true);
     else
-        PC_SWEEP_GLOBAL(uint64_t, false);
+        PC_SWEEP_GLOBAL_W(uint64_t, false);
+#undef PC_SWEEP_GLOBAL_W
 #undef PC_SWEEP_GLOBAL
     PC_LAUNCH_CHECK("sweep_global_kernel");
     return PC_OK;
@@ -2977,19 +3004,32 @@ extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
     g_last_error[0] = 0;
     if (!blocks_per_cu)
         return pc_fail(PC_EINVAL, "pc_sweep_occupancy: NULL output");
+    const uint32_t waves = state_mode & PC_SWEEP_WAVES_MASK;
+    state_mode &= ~PC_SWEEP_WAVES_MASK;
+    if (state_mode > PC_STATE_HBM || waves == PC_SWEEP_WAVES_MASK)
+        return pc_fail(PC_EINVAL, "pc_sweep_occupancy: unknown state_mode %u",
+                       state_mode | waves);
     int blocks = 0;
-    if (state_mode == PC_STATE_LDS ||
-        (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes)) {
+    const bool use_lds = state_mode == PC_STATE_LDS ||
+                         (state_mode == PC_STATE_AUTO && num_nodes <= kLdsMaxNodes);
+    const bool split = sweep_split(num_nodes, use_lds, waves);
+    if (use_lds) {
         if (num_nodes > kLdsMaxNodes)
             return pc_fail(PC_EINVAL, "pc_sweep_occupancy: too many nodes for LDS");
-        constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
+        if (split)
+            PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, sweep_lds_kernel<uint32_t, false, true>, 128,
+                sweep_lds_bytes<true>(num_nodes)));
+        else
+            PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, sweep_lds_kernel<uint32_t, false, false>, 64,
+                sweep_lds_bytes<false>(num_nodes)));
+    } else if (split) {
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_lds_kernel<uint32_t, false, SPLIT>, SPLIT ? 128 : 64,
-            sweep_lds_bytes<SPLIT>(num_nodes)));
+            &blocks, sweep_global_kernel<uint32_t, false, true>, 128, 0));
     } else {
-        constexpr bool SPLIT = PC_SWEEP_SPLIT != 0;
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_global_kernel<uint32_t, false, SPLIT>, SPLIT ? 128 : 64, 0));
+            &blocks, sweep_global_kernel<uint32_t, false, false>, 64, 0));
     }
     *blocks_per_cu = blocks;
     return PC_OK;

@@ -62,6 +62,10 @@ def test_argument_errors_without_gpu():
     assert rc == nat.PC_EINVAL
     with pytest.raises(ValueError):
         nat.check(rc, 'pc_sweep')
+    # both wave flags at once is not a mode
+    rc = lib.pc_sweep(16, 1, None, None, 0, None, 1, 1, None, 1, None, None, 0,
+                      0, nat.PC_SWEEP_WAVES_MASK, None, None)
+    assert rc == nat.PC_EINVAL and b'state_mode' in lib.pc_last_error()
     rc = lib.pc_binomial_window(10, 1.5, ctypes.byref(ctypes.c_int64()),
                                 ctypes.byref(ctypes.c_int64()))
     assert rc == nat.PC_EINVAL

@@ -57,12 +57,18 @@ def test_tuned_knobs_do_not_change_records(gpu, tmp_path, monkeypatch, L):
     ref = base.records.cpu().numpy().copy(), base.span_step.cpu().numpy().copy()
     choice = tune_sweep(eng, sample_runs=64, rounds=(1, 3, 8))
     assert choice['merge_rounds'] in (1, 3, 8)
-    modes = [nat.PC_STATE_HBM] + ([nat.PC_STATE_LDS] if eng.N <= 16384 else [])
+    places = [nat.PC_STATE_HBM] + ([nat.PC_STATE_LDS] if eng.N <= 16384 else [])
+    modes = [p | w for p in places
+             for w in (0, nat.PC_SWEEP_ONE_WAVE, nat.PC_SWEEP_TWO_WAVES)]
     for mode in modes:
         for r in (1, 5, 16):
             eng.state_mode, eng.merge_rounds = mode, r
-            res = eng.sweep(perm)
-            assert np.array_equal(res.records.cpu().numpy(), ref[0]), (mode, r)
-            assert np.array_equal(res.span_step.cpu().numpy(), ref[1]), (mode, r)
+            for chunks in (False, True):
+                res = eng.sweep(perm, chunks=chunks)
+                assert np.array_equal(res.records.cpu().numpy(), ref[0]), (mode, r)
+                assert np.array_equal(res.span_step.cpu().numpy(), ref[1]), (mode, r)
+                if chunks:
+                    assert np.array_equal(res.chunk_state.cpu().numpy(),
+                                          eng.chunk_scan(res).cpu().numpy()), (mode, r)
     # cached: a second tuning returns the stored choice
     assert tune_sweep(eng, sample_runs=64, rounds=(1, 3, 8)) == choice
