@@ -665,13 +665,15 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     uint32_t span_at = PC_NO_SPAN;
     uint32_t tag = 0;
 
-    // software pipeline: the permutation two batches ahead and the edge
+    // one wavefront per run: the permutation two batches ahead and the edge
     // endpoints one batch ahead are in flight while a batch is resolved
-    // (SPLIT: while the finder wave works on the batch before)
-    int32_t e_ahead = (64u + lane < M) ? perm[64u + lane] : 0;
+    int32_t e_ahead = 0;
     int2 uv_next = make_int2(0, 0);
-    if (lane < M)
-        uv_next = edges[perm[lane]];
+    if (!SPLIT) {
+        e_ahead = (64u + lane < M) ? perm[64u + lane] : 0;
+        if (lane < M)
+            uv_next = edges[perm[lane]];
+    }
     // codes of the next batch's endpoints, read one batch early: parent
     // links only ever move up the tree (a union links a root under a root,
     // halving replaces a parent by its grandparent), so a code read before
@@ -681,12 +683,35 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     uint32_t hint_u = 0xFFFFFFFFu, hint_v = 0xFFFFFFFFu;
 
     // SPLIT finder: roots of the batch at b0 into handoff slot (b0 / 64) & 1
-    // (u roots in words 0..63, v roots in 64..127 of the slot)
+    // (u roots in words 0..63, v roots in 64..127 of the slot).  Its own
+    // pipeline, issued at the start of a batch so the loads complete beside
+    // the finds: the permutation three batches ahead, the endpoints two
+    // batches ahead, and the codes of the next batch's endpoints (the hints
+    // above; with the state in HBM the finder is the slower wave and each
+    // hint saves a dependent read)
+    int2 f_uv1 = make_int2(0, 0), f_uv2 = make_int2(0, 0);
+    int32_t f_e3 = 0;
+    uint32_t f_hu = 0xFFFFFFFFu, f_hv = 0xFFFFFFFFu;
+    if (SPLIT && !resolver) {
+        if (lane < M)
+            f_uv1 = edges[perm[lane]];
+        if (64u + lane < M)
+            f_uv2 = edges[perm[64u + lane]];
+        f_e3 = (128u + lane < M) ? perm[128u + lane] : 0;
+    }
     auto find_batch = [&](uint32_t b0) {
-        const int2 uv = uv_next;
-        uv_next = (b0 + 64u + lane < M) ? edges[e_ahead] : make_int2(0, 0);
-        e_ahead = (b0 + 128u + lane < M) ? perm[b0 + 128u + lane] : 0;
-        uint32_t x = (uint32_t)uv.x, y = (uint32_t)uv.y, cx, cy;
+        const int2 uv = f_uv1;
+        const uint32_t hu = f_hu, hv = f_hv;
+        f_uv1 = f_uv2;
+        if (b0 + 64u + lane < M) {
+            f_hu = st.get((uint32_t)f_uv1.x);
+            f_hv = st.get((uint32_t)f_uv1.y);
+        }
+        f_uv2 = (b0 + 128u + lane < M) ? edges[f_e3] : make_int2(0, 0);
+        f_e3 = (b0 + 192u + lane < M) ? perm[b0 + 192u + lane] : 0;
+        uint32_t x = st.is_root(hu) ? (uint32_t)uv.x : hu;
+        uint32_t y = st.is_root(hv) ? (uint32_t)uv.y : hv;
+        uint32_t cx, cy;
         if (b0 + lane < M)
             find2(st, x, cx, y, cy);
         uint32_t *slot = handoff + ((b0 / 64u) & 1u) * 128u;
