@@ -591,9 +591,10 @@ def main(argv=None):
                          '-> chunk states -> exact microcanonical sums').format(
                              args.length, N, M, R),
             'length': args.length, 'runs': R, 'nodes': N, 'edges': M,
-            'parallelism': ('single GPU, one run per wavefront; sub-batches of '
-                            '{} runs, reduction overlapped with the next sweep '
-                            'on a second stream').format(pipe.sub),
+            'parallelism': ('single GPU, one run per workgroup of {} '
+                            'wavefront(s); sub-batches of {} runs, reduction '
+                            'overlapped with the next sweep on another '
+                            'stream').format(2 if N >= 4096 else 1, pipe.sub),
         },
         'roofline': {
             'kernel': 'sweep_lds_kernel' if N <= 16384 else 'sweep_global_kernel',

@@ -5,7 +5,8 @@
 // Pipeline for R runs on a graph with N nodes and M edges:
 //   permute (Philox-keyed Feistel, or MT19937 + numpy's shuffle for
 //   bit-exact RandomState permutations) -> sweep (union-find, one run per
-//   wavefront, merge records) -> chunk states (per-run state every 64
+//   workgroup of one or two wavefronts, merge records) -> chunk states
+//   (per-run state every 64
 //   steps: two passes over the records, or written by the sweep itself) ->
 //   microcanonical sums (exact Sigma x, Sigma x^2 over runs per step) or
 //   expand (per-run traces) -> canonical averages (binomial convolution on
