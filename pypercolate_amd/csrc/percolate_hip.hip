@@ -1177,6 +1177,52 @@ __global__ __launch_bounds__(64) void micro_sums_kernel(
         acc2_top[k] = 0;
     }
 
+    if constexpr (!WIDE) {
+        // N <= 2^14 and at most 2^16 runs per group bound every accumulator:
+        // max, #clusters - 1 and N - max are below 2^14 (squares below 2^28,
+        // group sums of squares below 2^44); the spanning flag is 0 or 1;
+        // moment 2 is below N^2 / 2 (square below 2^54), moment 3 below
+        // N^3 / 4 (group sum below 2^56), moment 4 below N^4 / 8.  So the
+        // narrow columns use 32- and 64-bit arithmetic, exact all the same.
+        uint32_t n1[4] = {0, 0, 0, 0};  // sum x: max, span, #clusters - 1, N - max
+        uint64_t n2[4] = {0, 0, 0, 0};  // sum x^2 of the same
+        uint64_t m1_2 = 0, m1_3 = 0;    // sum x: moments 2 and 3
+        u128 m1_4 = 0, m2_2 = 0, m2_3 = 0, m2_4 = 0;
+        for (uint32_t r = r0; r < r1; ++r) {
+            const RunState<false> st0 = load_state<false>(
+                chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS);
+            const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+            const RunState<false> st = advance<false>(st0, Deltas<false>(rec));
+            uint64_t x[7];
+            observables<false>(st, N, s >= span_step[r], x);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t v = (uint32_t)x[k];
+                n1[k] += v;
+                n2[k] += (uint64_t)(v * v);
+            }
+            const uint32_t v4 = (uint32_t)x[4];
+            m1_2 += v4;
+            m2_2 += (u128)((uint64_t)v4 * v4);
+            m1_3 += x[5];
+            m2_3 += (u128)x[5] * x[5];
+            m1_4 += x[6];
+            m2_4 += (u128)x[6] * x[6];
+        }
+        // column order of the sums: max, span, moments 0..4
+        const int col[4] = {0, 1, 2, 3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc1[col[k]] = n1[k];
+            acc2[col[k]] = n2[k];
+        }
+        acc1[4] = m1_2;
+        acc2[4] = m2_2;
+        acc1[5] = m1_3;
+        acc2[5] = m2_3;
+        acc1[6] = m1_4;
+        acc2[6] = m2_4;
+    } else {
     for (uint32_t r = r0; r < r1; ++r) {
         const RunState<WIDE> st0 = load_state<WIDE>(
             chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS);
@@ -1198,6 +1244,7 @@ __global__ __launch_bounds__(64) void micro_sums_kernel(
                 acc2[k] += (u128)x[k] * (u128)x[k];
             }
         }
+    }
     }
 
     if (valid) {
