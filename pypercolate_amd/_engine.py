@@ -243,15 +243,33 @@ def host_sums(sum1, sum2):
             limbs_to_int(sum2.cpu().numpy()))
 
 
+_PIPELINE_STREAMS = {}
+
+
+def _pipeline_streams(torch, device):
+    """The three streams every Pipeline on a device shares, created once
+    per process: HIP deals streams to the hardware queues in creation order,
+    so three fresh streams per pipeline would sooner or later share a queue
+    with each other (or the caller's) and serialise."""
+    key = torch.device(device).index
+    if key not in _PIPELINE_STREAMS:
+        with torch.cuda.device(device):
+            _PIPELINE_STREAMS[key] = [torch.cuda.Stream() for _ in range(3)]
+    return _PIPELINE_STREAMS[key]
+
+
 class Pipeline(object):
     """R runs of one graph through permute -> sweep -> chunk states -> exact
-    sums, in sub-batches on three streams so that the reduction of one
+    sums, in sub-batches on four streams so that the reduction of one
     sub-batch runs beside the sweep of the next (the sweep leaves most issue
     slots of a CU idle: its occupancy is bounded by LDS, four runs per CU at
-    N = 16384).  Stream order: permutations on one stream, sweeps on a
-    second (each waiting for its permutations), chunk states and sums on a
-    third (each waiting for its sweep, accumulating into one pair of sum
-    buffers).  HBM-state graphs get their chunk states from the sweep itself
+    N = 16384).  Stream order: permutations on the caller's stream, sweeps
+    alternating between two more (each waiting for its permutations; the
+    next sub-batch's runs fill the CUs that the last runs of a launch leave
+    idle), chunk states and sums on a fourth (each waiting for its sweep,
+    accumulating into one pair of sum buffers).  Four streams in all, one
+    per hardware queue of the process (HIP's default of four): streams that
+    shared a queue would serialise.  HBM-state graphs get their chunk states from the sweep itself
     (BondEngine.fuse_chunk_states).  All buffers are allocated up front.
 
     Permutations: Philox-keyed (pc_permute_philox, seed) by default, or
@@ -290,9 +308,11 @@ class Pipeline(object):
         self.sum2 = torch.empty((nat.PC_COLUMNS, S, 3), dtype=torch.int64,
                                 device=dev)
         wb = eng.workspace_bytes(self.sub)
-        self.workspace = (torch.empty((wb,), dtype=torch.uint8, device=dev)
-                          if wb else None)
-        self.streams = [torch.cuda.Stream() for _ in range(3)]
+        # one union-find workspace (HBM state) per sweep stream in use
+        nsub = (self.R + self.sub - 1) // self.sub
+        self.workspaces = [torch.empty((wb,), dtype=torch.uint8, device=dev)
+                           if wb else None for _ in range(min(2, nsub))]
+        self.streams = _pipeline_streams(torch, dev)
         self.sweep_events = []
 
     def run(self):
@@ -300,7 +320,8 @@ class Pipeline(object):
         torch = self.torch
         eng = self.engine
         main = torch.cuda.current_stream()
-        sp, ss, sr = self.streams
+        ss0, ss1, sr = self.streams
+        sp = main
         start = torch.cuda.Event()
         start.record(main)
         for s in self.streams:
@@ -308,8 +329,9 @@ class Pipeline(object):
         self.sweep_events = []
         first = True
         fused = eng.fuse_chunk_states()
-        for b0 in range(0, self.R, self.sub):
+        for i, b0 in enumerate(range(0, self.R, self.sub)):
             n = min(self.sub, self.R - b0)
+            ss = (ss0, ss1)[i & 1]
             perm = self.perm[b0:b0 + n]
             with torch.cuda.stream(sp):
                 if self.d_seeds is None:
@@ -324,7 +346,7 @@ class Pipeline(object):
                 e0.record(ss)
                 res = eng.sweep(perm, records=self.records[b0:b0 + n],
                                 span_step=self.span[b0:b0 + n],
-                                workspace=self.workspace,
+                                workspace=self.workspaces[i & 1],
                                 chunk_state=self.state[b0:b0 + n] if fused else None,
                                 chunks=fused)
                 e1 = torch.cuda.Event(enable_timing=True)
