@@ -549,8 +549,12 @@ def main(argv=None):
     t_wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(stop)
     ms_per_step = total_ms / args.steps
-    # HIP-event time of the sweep launches of the last step (same stream)
+    # HIP-event time of the sweep launches of the last step, each on the
+    # stream it runs on (two sweep streams: consecutive launches overlap, so
+    # the average launch duration also counts the time a launch shares the
+    # GPU with its neighbour; the span is first start to last end)
     sweep_step_ms = pipe.sweep_ms()
+    sweep_span_ms = pipe.sweep_span_ms()
     launches = len(pipe.sweep_events)
     sweep_avg_ms = sweep_step_ms / max(launches, 1)
 
@@ -592,9 +596,10 @@ def main(argv=None):
                              args.length, N, M, R),
             'length': args.length, 'runs': R, 'nodes': N, 'edges': M,
             'parallelism': ('single GPU, one run per workgroup of {} '
-                            'wavefront(s); sub-batches of {} runs, reduction '
-                            'overlapped with the next sweep on another '
-                            'stream').format(2 if N >= 4096 else 1, pipe.sub),
+                            'wavefront(s); sub-batches of {} runs (one '
+                            'residency), sweeps alternating between two '
+                            'streams, reductions beside them on a third').format(
+                                2 if N >= 4096 else 1, pipe.sub),
         },
         'roofline': {
             'kernel': 'sweep_lds_kernel' if N <= 16384 else 'sweep_global_kernel',
@@ -609,6 +614,12 @@ def main(argv=None):
             'kernel_ms': sweep_avg_ms,
             'launches_per_step': launches,
             'sweep_ms_per_step': sweep_step_ms,
+            # the sweep's throughput while it runs: all launches of the step
+            # over their span (launches overlap on two streams)
+            'span_ms': sweep_span_ms,
+            'achieved_over_span': alg_bytes * adds / (sweep_span_ms / 1e3) / 1e9,
+            'frac_over_span': (alg_bytes * adds / (sweep_span_ms / 1e3) / 1e9
+                               / HBM_PEAK_GBS),
         },
         'tuning': tuning,
         'sums_check': ok,

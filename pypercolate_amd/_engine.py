@@ -289,12 +289,11 @@ class Pipeline(object):
         dev = eng.device
         M, S = eng.M, eng.S
         if sub_batch is None:
-            # a whole number of residencies (a partial one leaves CUs idle at
-            # the end of every launch); two measured best at L = 128 (10.6-10.8
-            # ms per step of 10^4 runs, against 10.7-11.2 with three and 12.0
-            # with one)
+            # one residency (runs the GPU holds at once): with the sweeps on
+            # two streams, 9.3 ms per step of 10^4 runs at L = 128 against
+            # 9.4 with 1.5 residencies, 9.5 with two and 10.6 with three
             cus = torch.cuda.get_device_properties(dev).multi_processor_count
-            sub_batch = 2 * max(eng.occupancy(), 1) * cus
+            sub_batch = max(eng.occupancy(), 1) * cus
         self.sub = max(1, min(int(sub_batch), self.R))
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
         self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
@@ -366,8 +365,17 @@ class Pipeline(object):
         return self.sum1, self.sum2
 
     def sweep_ms(self):
-        """Summed HIP-event time of the sweeps of the last run()."""
+        """Summed HIP-event time of the sweeps of the last run() (launches
+        on the two sweep streams overlap, so this exceeds their span)."""
         return sum(a.elapsed_time(b) for a, b in self.sweep_events)
+
+    def sweep_span_ms(self):
+        """HIP-event time from the start of the first sweep of the last
+        run() to the end of its last sweep."""
+        if not self.sweep_events:
+            return 0.0
+        return max(self.sweep_events[0][0].elapsed_time(b)
+                   for _, b in self.sweep_events)
 
 
 class Weights(object):
