@@ -5,12 +5,12 @@ spanning_2d_grid(L), its buffers and its tuned sweep knobs, and runs for
 each L the pipeline permute (Philox) -> sweep -> chunk states -> exact
 microcanonical sums -> mean per step -> canonical averages at the given p
 (FP64 on the matrix cores, binomial weights generated inside the
-contraction).  The per-L pipelines are independent: they
-are issued on one stream each (default priority), forked from and joined
-back into the caller's stream with events, so the GPU overlaps the long
-large-L sweeps with the many short small-L ones.  ``capture()`` records the
-whole program once as a HIP graph (torch.cuda.CUDAGraph); ``run()`` then
-replays it with a single launch.  All buffers are allocated up front, so a
+contraction).  The per-L pipelines are independent: they are issued on
+three streams (default priority; the largest L, the next, and the rest in
+turn), forked from and joined back into the caller's stream with events, so
+the GPU overlaps the long large-L sweeps with the many short small-L ones.
+``capture()`` records the whole program once as a HIP graph
+(torch.cuda.CUDAGraph); ``run()`` then replays it with a single launch.  All buffers are allocated up front, so a
 replay allocates nothing.
 
 This is the batched form of the reference's jug study
@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _graph
 from . import _native as nat
-from ._engine import BondEngine, FusedPlan, micro_moments
+from ._engine import BondEngine, FusedPlan, _pipeline_streams, micro_moments
 from .autotune import tune_sweep
 
 
@@ -87,9 +87,16 @@ class FiniteSizeBatch(object):
             runs = [int(runs)] * len(lengths)
         self.plans = [_LengthPlan(int(L), int(r), seed, ps, tune)
                       for L, r in zip(lengths, runs)]
-        # largest lattices first: their sweeps are the critical path
+        # largest lattices first: their sweeps are the critical path.  Three
+        # streams besides the caller's (one per hardware queue of the
+        # process: streams that share a queue serialise): the largest L on
+        # the first, the next on the second, all smaller ones in turn on
+        # the third
         self.plans.sort(key=lambda p: -p.engine.M)
-        self.streams = [torch.cuda.Stream() for _ in self.plans]
+        self.streams = _pipeline_streams(torch, self.plans[0].engine.device)[
+            :min(3, len(self.plans))]
+        self.stream_of = [min(i, len(self.streams) - 1)
+                          for i in range(len(self.plans))]
         self.graph = None
 
     def _launch_all(self):
@@ -98,8 +105,10 @@ class FiniteSizeBatch(object):
         fork = torch.cuda.Event()
         fork.record(main)
         joins = []
-        for plan, stream in zip(self.plans, self.streams):
-            stream.wait_event(fork)
+        for s in self.streams:
+            s.wait_event(fork)
+        for i, plan in enumerate(self.plans):
+            stream = self.streams[self.stream_of[i]]
             with torch.cuda.stream(stream):
                 plan.launch()
                 done = torch.cuda.Event()
