@@ -12,6 +12,7 @@ import ctypes
 
 import numpy as np
 
+from . import _graph
 from . import _native as nat
 
 
@@ -40,7 +41,11 @@ class SweepResult(object):
 class BondEngine(object):
     """One percolation graph resident on the GPU."""
 
-    def __init__(self, graph):
+    def __init__(self, graph, relabel=None):
+        """relabel: store the union-find state of 2-D grid nodes in tiles
+        (_graph.locality_order) -- by default for graphs whose state lives
+        in HBM (N > 16384), where every random access moves a whole line.
+        The results do not depend on it."""
         self.torch = torch = nat.torch_cuda()
         self.lib = nat.load_library()
         self.graph = graph
@@ -49,10 +54,19 @@ class BondEngine(object):
         self.S = self.M + 1
         dev = torch.device('cuda', torch.cuda.current_device())
         self.device = dev
-        self.d_edges = torch.from_numpy(graph.edges.reshape(-1).copy()).to(dev)
+        if relabel is None:
+            relabel = self.N > 16384
+        order = _graph.locality_order(graph) if relabel else None
+        self.relabelled = order is not None
+        edges = graph.edges if order is None else order[graph.edges]
+        sides = graph.node_sides
+        if order is not None and sides is not None:
+            sides = np.empty_like(graph.node_sides)
+            sides[order] = graph.node_sides
+        self.d_edges = torch.from_numpy(
+            np.ascontiguousarray(edges, dtype=np.int32).reshape(-1)).to(dev)
         self.d_sides = (
-            None if graph.node_sides is None
-            else torch.from_numpy(graph.node_sides.copy()).to(dev)
+            None if sides is None else torch.from_numpy(sides.copy()).to(dev)
         )
         self.span_first = 1 if graph.span_on_first_merge else 0
         self.rec_bytes = self.lib.pc_record_bytes(self.N)

@@ -156,6 +156,30 @@ def from_percolation_graph(perc_graph, spanning_cluster=True,
     )
 
 
+def locality_order(graph, tile_rows=4, tile_cols=8):
+    """Node relabelling old -> new (int64 array) that stores the union-find
+    codes of 2-D grid nodes in tiles of tile_rows x tile_cols (32 codes =
+    one 128-byte line), so the two endpoints of most bonds share a line; or
+    None when the node labels are not pairs of integers.  The sweep's
+    results do not depend on node labels (only on cluster sizes and the
+    spanning sides), so any relabelling gives the same records."""
+    nodes = graph.nodes
+    if nodes is None or len(nodes) != graph.num_nodes or not len(nodes):
+        return None
+    try:
+        coords = np.array(nodes, dtype=np.int64)
+    except (TypeError, ValueError):
+        return None
+    if coords.ndim != 2 or coords.shape[1] != 2:
+        return None
+    a = coords[:, 0] - coords[:, 0].min()
+    b = coords[:, 1] - coords[:, 1].min()
+    key = np.lexsort((b % tile_cols, a % tile_rows, b // tile_cols, a // tile_rows))
+    new = np.empty(graph.num_nodes, dtype=np.int64)
+    new[key] = np.arange(graph.num_nodes, dtype=np.int64)
+    return new
+
+
 def square_lattice(length, spanning_cluster=True):
     """Closed form of percolation_graph(spanning_2d_grid(length)) with
     spanning_cluster=True (with spanning_cluster=False: the L x L grid

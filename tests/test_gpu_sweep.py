@@ -220,3 +220,18 @@ def test_sweep_chunk_states_equal_chunk_scan(gpu, length, mode):
     fused = res.chunk_state.cpu().numpy()
     scanned = eng.chunk_scan(res).cpu().numpy()
     assert np.array_equal(fused, scanned)
+
+
+@pytest.mark.parametrize('length', [130, 257])
+def test_relabelled_state_gives_same_records(gpu, length):
+    """HBM-state graphs store the union-find codes in 4 x 8 tiles of the
+    grid (BondEngine relabel): records, span steps and chunk states equal
+    those of the graph's own labels."""
+    g = _graph.square_lattice(length)
+    a, b = BondEngine(g, relabel=True), BondEngine(g, relabel=False)
+    assert a.relabelled and not b.relabelled
+    perm = a.device_permutations(5, 40)
+    ra, rb = a.sweep(perm, chunks=True), b.sweep(perm, chunks=True)
+    assert np.array_equal(ra.records.cpu().numpy(), rb.records.cpu().numpy())
+    assert np.array_equal(ra.span_step.cpu().numpy(), rb.span_step.cpu().numpy())
+    assert np.array_equal(ra.chunk_state.cpu().numpy(), rb.chunk_state.cpu().numpy())
