@@ -2178,8 +2178,12 @@ __global__ __launch_bounds__(256) void fused_reduce_kernel(
 // out[(s - g_lo) * (R * C) + r * C + c] for steps s in [g_lo, g_lo + g_len),
 // columns [has_spanning_cluster if spanning,] max_cluster_size,
 // moments[0..4] (moments as their uint64 values, like the reference's
-// structured array).  Block (chunk of 64 steps, run), lane = step.
+// structured array).
 // ---------------------------------------------------------------------------
+#ifndef WT_RUNS
+#define WT_RUNS 4
+#endif
+
 template <typename RecT, bool WIDE>
 __global__ __launch_bounds__(64) void window_traces_kernel(
     uint32_t N, uint32_t M, uint32_t R, const RecT *__restrict__ records,
@@ -2187,8 +2191,15 @@ __global__ __launch_bounds__(64) void window_traces_kernel(
     const uint64_t *__restrict__ chunk_state, int64_t g_lo, int64_t g_len,
     int spanning, double *__restrict__ out)
 {
+    // Block (chunk of 64 steps, group of WT_RUNS runs): lane = step while
+    // the observables are computed, then the tile goes out row by row, the
+    // group's WT_RUNS * cols values of a step contiguous in the output (one
+    // coalesced store per step instead of 64 scattered ones per run)
     using MT = typename Wide<WIDE>::T;
-    const uint32_t r = blockIdx.y, lane = threadIdx.x;
+    __shared__ double tile[64][WT_RUNS * 7];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t r0 = blockIdx.y * WT_RUNS;
+    const uint32_t nr = min((uint32_t)WT_RUNS, R - r0);
     const uint32_t C = (M + 63u) / 64u;
     // chunk index of this block within the steps 1..M (state after c*64)
     const int64_t first_chunk = g_lo > 0 ? (g_lo - 1) / 64 : 0;
@@ -2200,16 +2211,16 @@ __global__ __launch_bounds__(64) void window_traces_kernel(
         return;
     }
     const int64_t q = c * 64 + lane; // record index, step q + 1
-    const int64_t step = q + 1;
     const bool valid = q < (int64_t)M;
-    const RunState<WIDE> st0 = load_state<WIDE>(
-        chunk_state + ((size_t)r * C + (size_t)c) * PC_STATE_WORDS);
-    const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
-    const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
-    MT x[7];
-    observables<WIDE>(st, N, (uint64_t)step >= span_step[r], x);
-    if (valid && step >= g_lo && step < g_lo + g_len) {
-        double *row = out + (step - g_lo) * width + (int64_t)r * cols;
+    for (uint32_t j = 0; j < nr; ++j) {
+        const uint32_t r = r0 + j;
+        const RunState<WIDE> st0 = load_state<WIDE>(
+            chunk_state + ((size_t)r * C + (size_t)c) * PC_STATE_WORDS);
+        const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+        const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
+        MT x[7];
+        observables<WIDE>(st, N, (uint64_t)(q + 1) >= span_step[r], x);
+        double *row = tile[lane] + j * cols;
         int k = 0;
         if (spanning)
             row[k++] = (double)(uint64_t)x[1];
@@ -2217,9 +2228,19 @@ __global__ __launch_bounds__(64) void window_traces_kernel(
         for (int m = 0; m < 5; ++m)
             row[k++] = (double)(uint64_t)x[2 + m];
     }
-    if (c == first_chunk && lane == 0 && g_lo == 0) {
+    __syncthreads();
+    const uint32_t n = nr * (uint32_t)cols;
+    for (uint32_t i = 0; i < 64u; ++i) {
+        const int64_t step = c * 64 + i + 1;
+        if (step > (int64_t)M || step < g_lo || step >= g_lo + g_len)
+            continue;
+        double *dst = out + (step - g_lo) * width + (int64_t)r0 * cols;
+        if (lane < n)
+            dst[lane] = tile[i][lane];
+    }
+    if (c == first_chunk && g_lo == 0 && lane < nr) {
         // step 0: max 1, no spanning, moments N - 1
-        double *row = out + (int64_t)r * cols;
+        double *row = out + (int64_t)(r0 + lane) * cols;
         int k = 0;
         if (spanning)
             row[k++] = 0.0;
@@ -3129,7 +3150,8 @@ extern "C" int pc_window_traces(uint32_t num_nodes, uint32_t num_edges,
     const int64_t first_chunk = g_lo > 0 ? (g_lo - 1) / 64 : 0;
     const int64_t last_step = g_lo + g_len - 1;
     const int64_t last_chunk = last_step > 0 ? (last_step - 1) / 64 : 0;
-    const dim3 grid((uint32_t)(last_chunk - first_chunk + 1), num_runs);
+    const dim3 grid((uint32_t)(last_chunk - first_chunk + 1),
+                    (num_runs + WT_RUNS - 1u) / WT_RUNS);
     hipStream_t s = as_stream(stream);
     if (num_nodes <= kLdsMaxNodes)
         window_traces_kernel<uint32_t, false><<<grid, 64, 0, s>>>(
