@@ -308,6 +308,11 @@ class Pipeline(object):
             # 9.4 with 1.5 residencies, 9.5 with two and 10.6 with three
             cus = torch.cuda.get_device_properties(dev).multi_processor_count
             sub_batch = max(eng.occupancy(), 1) * cus
+            if seeds is not None:
+                # RandomState permutations: one wavefront per run walks its
+                # shuffle in sequence, so a launch needs many runs per SIMD
+                # to hide its latencies; they, not the sweeps, set the pace
+                sub_batch *= 4
         self.sub = max(1, min(int(sub_batch), self.R))
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
         self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
