@@ -90,6 +90,20 @@ def test_random_graph_no_spanning(gpu):
     _check_sums(eng, res, expect)
 
 
+@pytest.mark.parametrize('nodes,edges', [(6000, 18000), (20000, 30000)])
+def test_random_graph_two_wavefronts(gpu, nodes, edges):
+    """Random graphs past the two-wavefront threshold (4096 nodes), with the
+    state in LDS and in HBM: traces and sums equal the oracle's."""
+    import networkx as nx
+    gnx = nx.gnm_random_graph(nodes, edges, seed=nodes)
+    eng, g = _engine_for(gnx, False)
+    perms = oracle.randomstate_permutations(range(24), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms))
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+
+
 def test_L128_64_runs(gpu):
     """The L = 128 lattice (N = 16384, M = 32512), LDS state at its limit."""
     eng = BondEngine(_graph.square_lattice(128))
