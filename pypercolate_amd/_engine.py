@@ -294,6 +294,8 @@ class Pipeline(object):
         torch = self.torch = engine.torch
         self.engine = eng = engine
         self.R = int(num_runs)
+        if self.R < 1:
+            raise ValueError('a pipeline needs at least one run')
         self.seed = seed
         self.d_seeds = None
         if seeds is not None:
