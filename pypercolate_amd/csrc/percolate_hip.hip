@@ -859,6 +859,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                     span_at = min(span_at, base + max(l1, l2) + 1u);
                 }
             }
+            bool spans_here = false;
             if (go) {
                 uint32_t sa, fa, sb, fb;
                 st.decode(cu, sa, fa);
@@ -869,12 +870,10 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 st.set(small, big);
                 st.set(big, st.root(sa + sb, f == 3u ? 0u : f));
                 rec = pack_record<RecT>(sa, sb);
+                spans_here = span_first || f == 3u;
             }
             {
-                uint32_t fa, fb, sz;
-                st.decode(cu, sz, fa);
-                st.decode(cv, sz, fb);
-                const uint64_t spans = __ballot(go && (span_first || (fa | fb) == 3u));
+                const uint64_t spans = __ballot(spans_here);
                 if (spans)
                     span_at = min(span_at, base + (uint32_t)__builtin_ctzll(spans) + 1u);
             }
