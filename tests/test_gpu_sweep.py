@@ -249,3 +249,30 @@ def test_relabelled_state_gives_same_records(gpu, length):
     assert np.array_equal(ra.records.cpu().numpy(), rb.records.cpu().numpy())
     assert np.array_equal(ra.span_step.cpu().numpy(), rb.span_step.cpu().numpy())
     assert np.array_equal(ra.chunk_state.cpu().numpy(), rb.chunk_state.cpu().numpy())
+
+
+@pytest.mark.parametrize('kind,length', [('grid', 2), ('grid', 5), ('grid', 9),
+                                         ('grid', 33), ('chain', 2), ('chain', 3),
+                                         ('chain', 200)])
+@pytest.mark.parametrize('placement', ['lds', 'hbm'])
+def test_two_wavefronts_small_graphs(gpu, kind, length, placement):
+    """The two-wavefront sweep forced onto small graphs, whose only batch
+    or last batch is partial: traces and sums equal the oracle's, and the
+    chunk states it writes equal the separate pass."""
+    from pypercolate_amd import _native as nat
+    from tests.helpers import spanning_1d_chain
+    if kind == 'grid':
+        ga, nxg = _graph.square_lattice(length), spanning_2d_grid(length)
+    else:
+        ga, nxg = _graph.linear_chain(length), spanning_1d_chain(length)
+    g = oracle.percolation_graph_arrays(nxg, True)
+    eng = BondEngine(ga)
+    eng.state_mode = nat.PC_SWEEP_TWO_WAVES | (
+        nat.PC_STATE_LDS if placement == 'lds' else nat.PC_STATE_HBM)
+    perms = oracle.randomstate_permutations(range(12), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms), chunks=True)
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+    assert np.array_equal(res.chunk_state.cpu().numpy(),
+                          eng.chunk_scan(res).cpu().numpy())
