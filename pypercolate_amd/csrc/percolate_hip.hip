@@ -429,22 +429,23 @@ __global__ __launch_bounds__(64) void permute_randomstate_kernel(
 // the run's spanning step and not stored (spanning is monotone).
 // ---------------------------------------------------------------------------
 
-// 16-bit codes in LDS for N <= 16384: c < N parent; c >= N root with
-// c - N = f * N + (size - 1).  The largest code, 4N - 1, fits 16 bits.
+// 16-bit codes in LDS for N <= 16384, with P = 2^shift the least power of
+// two >= N: c < N parent; c >= P root with c = (1 + f) * P + (size - 1).
+// The largest code, 4P - 1, fits 16 bits; decoding is a shift and a mask
+// (the resolver's rounds are bound by the instructions they issue).
 struct LdsState {
     uint16_t *code;
-    uint32_t N;
+    uint32_t P, shift;
     __device__ __forceinline__ uint32_t get(uint32_t i) const { return code[i]; }
     __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { code[i] = (uint16_t)v; }
-    __device__ __forceinline__ bool is_root(uint32_t v) const { return v >= N; }
-    __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return N * (1u + f) + size - 1u; }
+    __device__ __forceinline__ bool is_root(uint32_t v) const { return v >= P; }
+    __device__ __forceinline__ uint32_t root(uint32_t size, uint32_t f) const { return ((1u + f) << shift) + size - 1u; }
     // LDS operations of one wavefront complete in order
     __device__ __forceinline__ void sync() const {}
     __device__ __forceinline__ void decode(uint32_t v, uint32_t &size, uint32_t &f) const
     {
-        const uint32_t j = v - N;
-        f = j >= 2u * N ? 2u : (j >= N ? 1u : 0u);
-        size = j - (j >= 2u * N ? 2u * N : (j >= N ? N : 0u)) + 1u;
+        f = (v >> shift) - 1u;
+        size = (v & (P - 1u)) + 1u;
     }
 };
 
@@ -485,7 +486,13 @@ __device__ __forceinline__ void unpack_record(uint64_t r, uint32_t &a, uint32_t 
 
 // Two interleaved finds with path halving.  Concurrent halving by other
 // lanes of the wave only ever replaces a parent by an ancestor, so every
-// lane still reaches its true root.
+// lane still reaches its true root.  With the state in LDS each pass is
+// branch-free: both grandparents are read together, then (after the
+// halving stores) both new nodes' codes, so a pass is two LDS round trips
+// and about half the instructions of a branch per element (which also
+// waits for each read in turn).  A finished element re-reads its own node,
+// which costs nothing in LDS; in HBM those reads would be traffic, so the
+// global state keeps the branches.
 template <class St>
 __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
                                       uint32_t &y, uint32_t &cy)
@@ -493,6 +500,25 @@ __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
     cx = st.get(x);
     cy = st.get(y);
     bool rx = st.is_root(cx), ry = st.is_root(cy);
+    if constexpr (std::is_same<St, LdsState>::value) {
+        while (!(rx && ry)) {
+            const uint32_t gx = st.get(rx ? x : cx);
+            const uint32_t gy = st.get(ry ? y : cy);
+            const bool gxr = st.is_root(gx), gyr = st.is_root(gy);
+            if (!rx && !gxr)
+                st.set(x, gx);
+            if (!ry && !gyr)
+                st.set(y, gy);
+            x = rx ? x : (gxr ? cx : gx);
+            y = ry ? y : (gyr ? cy : gy);
+            const uint32_t nx = st.get(x), ny = st.get(y);
+            cx = rx ? cx : nx;
+            cy = ry ? cy : ny;
+            rx = st.is_root(cx);
+            ry = st.is_root(cy);
+        }
+        return;
+    }
     while (!(rx && ry)) {
         const uint32_t gx = rx ? cx : st.get(cx);
         const uint32_t gy = ry ? cy : st.get(cy);
@@ -845,8 +871,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                     // (size on u's side, size on v's side), as every merge
                     st.set(x, hub);
                     const uint32_t sh_now = s_hub + incl - sx;
-                    rec = at_u ? pack_record<RecT>(sh_now, sx)
-                               : pack_record<RecT>(sx, sh_now);
+                    rec = pack_record<RecT>(at_u ? sh_now : sx, at_u ? sx : sh_now);
                 }
                 if (lane == first)
                     st.set(hub, st.root(s_hub + total, f_all == 3u ? 0u : f_all));
@@ -961,7 +986,8 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
     extern __shared__ uint32_t lds_words[];
     const uint32_t r = blockIdx.x;
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
-    const LdsState st{(uint16_t *)(lds_words + sweep_lds_words<SPLIT>()), N};
+    const uint32_t shift = N > 1u ? 32u - (uint32_t)__builtin_clz(N - 1u) : 0u;
+    const LdsState st{(uint16_t *)(lds_words + sweep_lds_words<SPLIT>()), 1u << shift, shift};
     sweep_run<LdsState, RecT, CHUNKS, SPLIT>(
         st, N, M, edges, sides, span_first != 0, rounds, lds_words,
         lds_words + CLAIM_SLOTS, perm + (int64_t)r * perm_stride,
