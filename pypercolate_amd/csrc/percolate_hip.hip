@@ -608,13 +608,14 @@ __device__ __forceinline__ T write_lane(T v, uint32_t lane, T old)
 }
 
 // Conflict table for the parallel merge rounds: CLAIM_SLOTS words in LDS,
-// indexed by a multiplicative hash of a root.
+// indexed by a multiplicative hash of a root's low 24 bits (a full-rate
+// 24-bit multiply; a collision only blocks a merge for a round).
 #define CLAIM_BITS 10
 #define CLAIM_SLOTS (1u << CLAIM_BITS)
 
 __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 {
-    return (root * 2654435761u) >> (32 - CLAIM_BITS);
+    return ((root & 0xFFFFFFu) * 0x9E3779u) >> (32 - CLAIM_BITS);
 }
 
 // One run on one wavefront.  Bonds are taken 64 at a time (lane = step):
@@ -806,13 +807,14 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         uint64_t pending = __ballot(cand);
         RecT rec = 0;
 
+        if (tag >= 0xFFFFFFFFu - 65u * 64u) {
+            // this batch's round tags (at most 64 rounds) could wrap: clear
+            // the table and restart
+            for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
+                claims[i] = 0u;
+            tag = 0u;
+        }
         for (uint32_t k = 0; k < rounds && pending; ++k) {
-            if (tag >= 0xFFFFFF80u) {
-                // the round tags would wrap: clear the table and restart
-                for (uint32_t i = lane; i < CLAIM_SLOTS; i += 64u)
-                    claims[i] = 0u;
-                tag = 0u;
-            }
             tag += 64u;
             const uint32_t mine = tag + 63u - lane;
             const uint32_t su = claim_slot(ru), sv = claim_slot(rv);
