@@ -28,8 +28,12 @@ known-answer tests, the tutorial traces, brute-force connected components
 and closed-form end states; the binomial weights against scipy.stats.binom;
 the Chan merge against numpy mean/variance; finalize against scipy's t CDF;
 Philox against the Random123 known-answer vectors.  ``online_variance`` is
-pinned to tolerance only: simoa's exact floating-point operation order is
-parity unpinned (simoa is absent here, and unpinned in requirements.txt).
+pinned by the closed form (numpy's two-pass mean and sum of squared
+deviations of the concatenated samples) and by associativity under
+re-bracketing (mean to 1e-12, sum of squares to 1e-10): FP64 merges
+agree to rounding.  What no check here
+fixes is simoa's own operation order, hence its last bits (simoa is absent
+here, and unpinned in requirements.txt).
 """
 
 import ctypes
