@@ -668,9 +668,10 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           uint32_t *__restrict__ span_out,
                                           uint64_t *__restrict__ chunk_out)
 {
-    // the sweep is a latency-bound chain: let its wave win instruction
-    // arbitration over waves of other kernels sharing the SIMD (the
-    // reductions of the previous sub-batch run beside it)
+    // the sweep's pace is the resolver's dependent instruction chain: let
+    // its waves win instruction arbitration over waves of other kernels
+    // sharing the SIMD (the reductions of the previous sub-batch run beside
+    // it).  The finder keeps the same priority: lowering it measured slower.
     __builtin_amdgcn_s_setprio(3);
     // HBM-state graphs (N > 16384) need 128-bit S3 and S4
     constexpr bool WIDE = std::is_same<St, GlobalState>::value;
