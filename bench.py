@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # permutation read from HBM plus six 2-byte LDS accesses (two parent reads
 # for the finds, two root-size reads, one parent and one size write)
 ALG_BYTES_PER_ADD_LDS = 16.0
+# the same six accesses with the state in HBM (4-byte codes)
 ALG_BYTES_PER_ADD_HBM = 28.0
 
 
