@@ -222,11 +222,6 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
 #define PERM_ROUNDS 8
 #define PERM_BLOCK 256
 #define PERM_ITEMS 16
-// elements per thread mapped together (more measured slower: the kernel is
-// bound by the table reads, not by their latency)
-#ifndef PERM_ILP
-#define PERM_ILP 1
-#endif
 #define PERM_TAG 0x7065726du
 
 __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
@@ -254,49 +249,33 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
 
     int32_t *out = perm + (int64_t)blockIdx.y * stride;
     const uint32_t base = blockIdx.x * (PERM_BLOCK * PERM_ITEMS);
-    // PERM_ILP elements per thread at a time, their round chains interleaved
-    for (int k0 = 0; k0 < PERM_ITEMS; k0 += PERM_ILP) {
-        uint32_t y[PERM_ILP];
-        bool need[PERM_ILP];
-        bool any = false;
+    // Each lane maps its PERM_ITEMS elements in turn, one Feistel pass per
+    // trip: an image outside [0, M) is mapped again (cycle walking), an
+    // image inside is stored and the lane moves on to its next element.  A
+    // pass lands inside with probability M / 2^(2h), as low as 1/2, so a
+    // loop that waited for every lane of the wave before taking the next
+    // element would run about log2(64) + 1 passes per element instead of
+    // about two; here a lane that finished starts its next element while
+    // the others walk.
+    uint32_t k = 0;
+    uint32_t i = base + threadIdx.x;
+    uint32_t y = i;
+    bool active = i < M;
+    while (active) {
+        uint32_t L = y >> h, R = y & mask;
 #pragma unroll
-        for (int e = 0; e < PERM_ILP; ++e) {
-            y[e] = base + (uint32_t)(k0 + e) * PERM_BLOCK + threadIdx.x;
-            need[e] = y[e] < M;
-            any = any || need[e];
+        for (int j = 0; j < PERM_ROUNDS; ++j) {
+            const uint32_t t = L ^ table[((uint32_t)j << h) | R];
+            L = R;
+            R = t;
         }
-        // cycle walking: an element whose image lies outside [0, M) is
-        // mapped again (its chain is computed alongside the others')
-        while (any) {
-            uint32_t L[PERM_ILP], R[PERM_ILP];
-#pragma unroll
-            for (int e = 0; e < PERM_ILP; ++e) {
-                L[e] = y[e] >> h;
-                R[e] = y[e] & mask;
-            }
-#pragma unroll
-            for (int j = 0; j < PERM_ROUNDS; ++j)
-#pragma unroll
-                for (int e = 0; e < PERM_ILP; ++e) {
-                    const uint32_t t = L[e] ^ table[((uint32_t)j << h) | R[e]];
-                    L[e] = R[e];
-                    R[e] = t;
-                }
-            any = false;
-#pragma unroll
-            for (int e = 0; e < PERM_ILP; ++e) {
-                if (need[e]) {
-                    y[e] = (L[e] << h) | R[e];
-                    need[e] = y[e] >= M;
-                }
-                any = any || need[e];
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < PERM_ILP; ++e) {
-            const uint32_t i = base + (uint32_t)(k0 + e) * PERM_BLOCK + threadIdx.x;
-            if (i < M)
-                out[i] = (int32_t)y[e];
+        y = (L << h) | R;
+        if (y < M) {
+            out[i] = (int32_t)y;
+            ++k;
+            i = base + k * PERM_BLOCK + threadIdx.x;
+            y = i;
+            active = k < (uint32_t)PERM_ITEMS && i < M;
         }
     }
 }
