@@ -57,8 +57,10 @@ int pc_record_bytes(uint32_t num_nodes);
 /*
  * Device permutations (performance mode).  Fills d_perm[r * perm_stride + i]
  * for r < num_runs, i < num_edges with a pseudo-random permutation of
- * 0..num_edges-1 for run first_run + r: a cycle-walking Feistel network whose
- * eight round functions are tables drawn from Philox4x32-10 keyed by seed.
+ * 0..num_edges-1 for run first_run + r: a cycle-walking Feistel network on
+ * ceil(log2 num_edges) bits (halves of floor and ceil of half the bits, each
+ * round xoring a table lookup of one half into the other) whose eight round
+ * functions are tables drawn from Philox4x32-10 keyed by seed.
  * Replaces the host-side numpy RandomState(seed).permutation(num_edges) of
  * percolate/hpc.py:195,206 when bit-exactness with numpy is not required.
  */

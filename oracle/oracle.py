@@ -417,16 +417,19 @@ PERM_TAG = 0x7065726d
 
 
 def feistel_permutation(seed, run, num_edges):
-    """The permutation pc_permute_philox draws for one run (Python ints)."""
+    """The permutation pc_permute_philox draws for one run (Python ints):
+    an alternating Feistel network on b = ceil(log2 M) bits (high half of
+    floor(b/2) bits, low half of the rest), cycle-walked into [0, M)."""
     M = int(num_edges)
     if M == 0:
         return np.zeros(0, dtype=np.int64)
     bits = 1
     while (1 << bits) < M:
         bits += 1
-    h = (bits + 1) // 2
-    mask = (1 << h) - 1
-    entries = PERM_ROUNDS << h
+    c = (bits + 1) // 2
+    a = bits - c
+    cmask, amask = (1 << c) - 1, (1 << a) - 1
+    entries = PERM_ROUNDS << c
     table = np.zeros(entries, dtype=np.int64)
     for g in range((entries + 7) // 8):
         words = philox4x32_10((g, run & MASK32, run >> 32, PERM_TAG),
@@ -434,15 +437,17 @@ def feistel_permutation(seed, run, num_edges):
         for j in range(8):
             e = g * 8 + j
             if e < entries:
-                table[e] = (words[j >> 1] >> (16 * (j & 1))) & mask
+                half = (words[j >> 1] >> (16 * (j & 1))) & 0xFFFF
+                table[e] = half & (cmask if (e >> c) & 1 else amask)
     out = np.empty(M, dtype=np.int64)
     for i in range(M):
         y = i
         while True:
-            L, R = y >> h, y & mask
-            for j in range(PERM_ROUNDS):
-                L, R = R, L ^ int(table[(j << h) | R])
-            y = (L << h) | R
+            L, R = y >> c, y & cmask
+            for j in range(0, PERM_ROUNDS, 2):
+                L ^= int(table[(j << c) | R])
+                R ^= int(table[((j + 1) << c) | L])
+            y = (L << c) | R
             if y < M:
                 break
         out[i] = y

@@ -213,25 +213,34 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
 }
 
 // ---------------------------------------------------------------------------
-// Device permutations: a balanced Feistel network on 2h bits (2^(2h) >= M)
-// with PERM_ROUNDS random round tables T_j : [0, 2^h) -> [0, 2^h), drawn
-// per run from Philox4x32-10(counter = (table word, run lo, run hi, tag),
-// key = seed).  Cycle walking restricts the bijection of [0, 2^(2h)) to
-// [0, M).  perm[i] = edge index added at step i + 1.
+// Device permutations: a Feistel network on b = ceil(log2 M) bits, split
+// into a high half L of a = floor(b / 2) bits and a low half R of
+// c = b - a bits, with PERM_ROUNDS random round tables drawn per run from
+// Philox4x32-10(counter = (table word, run lo, run hi, tag), key = seed).
+// Even rounds j set L ^= T_j[R] (a-bit values), odd rounds R ^= T_j[L]
+// (c-bit values): each round is invertible, so the network is a bijection
+// of [0, 2^b), and cycle walking restricts it to [0, M).  (For even b this
+// equals the swapping form (L, R) -> (R, L ^ T_j[R]) after any even number
+// of rounds; an odd b no longer needs a domain twice as large.)
+// perm[i] = edge index added at step i + 1.
 // ---------------------------------------------------------------------------
 #define PERM_ROUNDS 8
 #define PERM_BLOCK 256
+#ifndef PERM_ITEMS
 #define PERM_ITEMS 16
+#endif
 #define PERM_TAG 0x7065726du
 
 __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
     uint32_t seed_lo, uint32_t seed_hi, uint64_t first_run, uint32_t M,
-    uint32_t h, int32_t *__restrict__ perm, int64_t stride)
+    uint32_t bits, int32_t *__restrict__ perm, int64_t stride)
 {
+    // round j's table holds 2^c entries (odd rounds use the first 2^a)
     extern __shared__ uint16_t table[];
     const uint64_t run = first_run + blockIdx.y;
-    const uint32_t entries = PERM_ROUNDS << h;
-    const uint32_t mask = (1u << h) - 1u;
+    const uint32_t c = (bits + 1u) / 2u, a = bits - c;
+    const uint32_t entries = PERM_ROUNDS << c;
+    const uint32_t cmask = (1u << c) - 1u, amask = (1u << a) - 1u;
 
     for (uint32_t g = threadIdx.x; g * 8u < entries; g += PERM_BLOCK) {
         const u32x4 o = philox4x32_10(
@@ -242,7 +251,8 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
         for (int j = 0; j < 8; ++j) {
             const uint32_t e = g * 8u + (uint32_t)j;
             if (e < entries)
-                table[e] = (uint16_t)((words[j >> 1] >> (16 * (j & 1))) & mask);
+                table[e] = (uint16_t)((words[j >> 1] >> (16 * (j & 1))) &
+                                      (((e >> c) & 1u) ? cmask : amask));
         }
     }
     __syncthreads();
@@ -252,7 +262,7 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
     // Each lane maps its PERM_ITEMS elements in turn, one Feistel pass per
     // trip: an image outside [0, M) is mapped again (cycle walking), an
     // image inside is stored and the lane moves on to its next element.  A
-    // pass lands inside with probability M / 2^(2h), as low as 1/2, so a
+    // pass lands inside with probability M / 2^b, as low as 1/2, so a
     // loop that waited for every lane of the wave before taking the next
     // element would run about log2(64) + 1 passes per element instead of
     // about two; here a lane that finished starts its next element while
@@ -262,14 +272,13 @@ __global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
     uint32_t y = i;
     bool active = i < M;
     while (active) {
-        uint32_t L = y >> h, R = y & mask;
+        uint32_t L = y >> c, R = y & cmask;
 #pragma unroll
-        for (int j = 0; j < PERM_ROUNDS; ++j) {
-            const uint32_t t = L ^ table[((uint32_t)j << h) | R];
-            L = R;
-            R = t;
+        for (uint32_t j = 0; j < PERM_ROUNDS; j += 2u) {
+            L ^= table[(j << c) | R];
+            R ^= table[((j + 1u) << c) | L];
         }
-        y = (L << h) | R;
+        y = (L << c) | R;
         if (y < M) {
             out[i] = (int32_t)y;
             ++k;
@@ -2544,13 +2553,12 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
     uint32_t bits = 1;
     while ((1ull << bits) < num_edges)
         ++bits;
-    const uint32_t h = (bits + 1u) / 2u;
-    const size_t lds = (size_t)(PERM_ROUNDS << h) * sizeof(uint16_t);
+    const size_t lds = (size_t)(PERM_ROUNDS << ((bits + 1u) / 2u)) * sizeof(uint16_t);
     const dim3 grid((num_edges + PERM_BLOCK * PERM_ITEMS - 1) /
                         (PERM_BLOCK * PERM_ITEMS),
                     num_runs);
     permute_kernel<<<grid, PERM_BLOCK, lds, as_stream(stream)>>>(
-        (uint32_t)seed, (uint32_t)(seed >> 32), first_run, num_edges, h,
+        (uint32_t)seed, (uint32_t)(seed >> 32), first_run, num_edges, bits,
         d_perm, perm_stride);
     PC_LAUNCH_CHECK("permute_kernel");
     return PC_OK;

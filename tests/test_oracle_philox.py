@@ -1,6 +1,7 @@
 """Pins the oracle's restatement of the device-permutation algorithm:
 Philox4x32-10 against the Random123 known-answer vectors, and the
-cycle-walking Feistel network as a bijection."""
+cycle-walking Feistel network as a bijection that, for an even bit count,
+equals the swapping form of the same network."""
 import numpy as np
 import pytest
 
@@ -33,3 +34,49 @@ def test_feistel_runs_differ():
     b = oracle.feistel_permutation(42, 1, 480)
     c = oracle.feistel_permutation(43, 0, 480)
     assert (a != b).mean() > 0.9 and (a != c).mean() > 0.9
+
+
+def _swapping_feistel(seed, run, M):
+    """The same network in the swapping form (L, R) -> (R, L ^ T_j[R]) on
+    two halves of b / 2 bits, for an even bit count b: after an even number
+    of rounds it must equal the oracle's alternating form."""
+    bits = max(1, (M - 1).bit_length())
+    assert bits % 2 == 0
+    h = bits // 2
+    mask = (1 << h) - 1
+    entries = oracle.PERM_ROUNDS << h
+    table = []
+    for g in range((entries + 7) // 8):
+        words = oracle.philox4x32_10((g, run & 0xFFFFFFFF, run >> 32, oracle.PERM_TAG),
+                                     (seed & 0xFFFFFFFF, seed >> 32))
+        for j in range(8):
+            table.append((words[j >> 1] >> (16 * (j & 1))) & mask)
+    out = []
+    for i in range(M):
+        y = i
+        while True:
+            L, R = y >> h, y & mask
+            for j in range(oracle.PERM_ROUNDS):
+                L, R = R, L ^ table[(j << h) | R]
+            y = (L << h) | R
+            if y < M:
+                break
+        out.append(y)
+    return out
+
+
+@pytest.mark.parametrize('M', [3, 4, 12, 200, 256, 3000])
+def test_feistel_even_bits_equals_swapping_form(M):
+    for run in (0, 9):
+        assert oracle.feistel_permutation(77, run, M).tolist() == \
+            _swapping_feistel(77, run, M)
+
+
+@pytest.mark.parametrize('M', [2, 5, 480, 32512])
+def test_feistel_odd_bits_is_permutation(M):
+    """Odd bit counts (halves of unequal size; the L = 128 lattice has
+    b = 15) still give a permutation of 0..M-1."""
+    bits = max(1, (M - 1).bit_length())
+    assert bits % 2 == 1
+    perm = oracle.feistel_permutation(5, 3, M)
+    assert sorted(perm.tolist()) == list(range(M))
