@@ -127,8 +127,10 @@ def test_global_state(gpu, length):
     _check_sums(eng, res, expect)
 
 
-@pytest.mark.parametrize('M', [1, 12, 480, 5000])
+@pytest.mark.parametrize('M', [1, 12, 480, 5000, 32512, 65536, 70000])
 def test_device_permutation_matches_oracle(gpu, M):
+    """Odd and even bit counts (1, 4, 9, 13, 15: the L = 128 lattice, 16 and
+    17 bits), against the oracle's restatement."""
     eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
         [np.arange(M), np.arange(1, M + 1)], axis=1)))
     d = eng.device_permutations(seed=1234, num_runs=3, first_run=5)
