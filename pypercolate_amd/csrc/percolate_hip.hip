@@ -370,7 +370,11 @@ __global__ __launch_bounds__(64) void permute_randomstate_kernel(
     }
     __syncthreads();
 
-    uint32_t pos = MT_N;
+    // The draws do not depend on the array: the wave tempers the next 64
+    // words of the state at once (lane k holds word base + k) and each draw
+    // is a lane read, so the only LDS round trip per step is the swap's.
+    uint32_t pos = MT_N, base = MT_N;
+    uint32_t buf = 0;
     uint32_t i = n ? n - 1u : 0u;
     while (i >= 1u) {
         uint32_t mask = i;
@@ -386,8 +390,14 @@ __global__ __launch_bounds__(64) void permute_randomstate_kernel(
                 mt_twist(mt);
                 __syncthreads();
                 pos = 0;
+                base = MT_N;
             }
-            v = mt_temper(uniform(mt[pos])) & mask;
+            if (pos - base >= 64u) {
+                // refill (base = MT_N after a twist makes pos - base wrap)
+                base = pos;
+                buf = base + lane < MT_N ? mt_temper(mt[base + lane]) : 0u;
+            }
+            v = read_lane(buf, pos - base) & mask;
             ++pos;
         } while (v > i);
         if (lane == 0) {
