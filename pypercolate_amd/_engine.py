@@ -308,6 +308,8 @@ class Pipeline(object):
             # one residency (runs the GPU holds at once): with the sweeps on
             # two streams, 9.3 ms per step of 10^4 runs at L = 128 against
             # 9.4 with 1.5 residencies, 9.5 with two and 10.6 with three
+            # (an earlier, slower sweep; smaller first or last sub-batches
+            # measured slower with the current one too)
             cus = torch.cuda.get_device_properties(dev).multi_processor_count
             sub_batch = max(eng.occupancy(), 1) * cus
             if seeds is not None:
