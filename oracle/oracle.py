@@ -10,6 +10,7 @@ step, citing the lines it follows:
 * ``percolation_graph_arrays``  percolate/percolate.py:31-100 (networkx 3 API)
 * ``bond_runs``                 percolate/hpc.py:194-307 via nz_oracle.c
 * ``microcanonical_sums``       exact sums over runs (nz_oracle.c)
+* ``bond_sums``                 sweep + exact sums without traces (nz_oracle.c)
 * ``binomial_pmf``              percolate/percolate.py:1067-1109
 * ``canonical_statistics``      percolate/hpc.py:443-515
 * ``initialize_canonical_averages``  percolate/hpc.py:561-635
@@ -65,9 +66,14 @@ def _load():
         lib.nzo_bond_runs.restype = ctypes.c_int
         lib.nzo_bond_runs.argtypes = [i64, i64, i64, vp, ctypes.c_int, i64,
                                       vp, i64, vp, vp, vp, vp, vp, vp, vp,
-                                      ctypes.c_int]
-        lib.nzo_microcanonical_sums.restype = None
-        lib.nzo_microcanonical_sums.argtypes = [i64, i64, vp, vp, vp, vp, vp]
+                                      vp, ctypes.c_int]
+        lib.nzo_microcanonical_sums.restype = ctypes.c_int
+        lib.nzo_microcanonical_sums.argtypes = [i64, i64, vp, vp, vp, vp, vp,
+                                                vp]
+        lib.nzo_bond_sums.restype = ctypes.c_int
+        lib.nzo_bond_sums.argtypes = [i64, i64, i64, vp, ctypes.c_int, i64,
+                                      vp, i64, vp, vp, vp, vp, vp,
+                                      ctypes.c_int, ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -131,11 +137,13 @@ def _edge_span_attributes(graph):
 # ---------------------------------------------------------------------------
 # Sweep (nz_oracle.c)
 # ---------------------------------------------------------------------------
-def bond_runs(g, perms, threads=None):
+def bond_runs(g, perms, threads=None, exact=False):
     """Traces of runs with the given permutations (R, M).
 
     Returns dict edge (R,S) u32, has_spanning_cluster (R,S) bool,
-    max_cluster_size (R,S) u32, moments (R,S,5) u64; S = M + 1.
+    max_cluster_size (R,S) u32, moments (R,S,5) u64 (the reference's values,
+    mod 2**64); S = M + 1.  With exact=True also moments_hi (R,S,5) u64, the
+    high words of the exact 128-bit moments.
     """
     lib = _load()
     M = g['num_edges']
@@ -147,6 +155,7 @@ def bond_runs(g, perms, threads=None):
     span = np.zeros((R, S), dtype=np.uint8)
     mx = np.zeros((R, S), dtype=np.uint32)
     mom = np.zeros((R, S, 5), dtype=np.uint64)
+    mom_hi = np.zeros((R, S, 5), dtype=np.uint64) if exact else None
     edges = np.ascontiguousarray(g['edges'], dtype=np.int64)
     aux_side = np.ascontiguousarray(g['aux_side'], dtype=np.int32)
     aux_edges = np.ascontiguousarray(g['aux_edges'], dtype=np.int64)
@@ -154,26 +163,78 @@ def bond_runs(g, perms, threads=None):
     rc = lib.nzo_bond_runs(
         R, g['num_nodes'], M, _p(edges), int(g['spanning']), aux_side.size,
         _p(aux_side), aux_edge_side.size, _p(aux_edges), _p(aux_edge_side),
-        _p(perms), _p(edge), _p(span), _p(mx), _p(mom),
+        _p(perms), _p(edge), _p(span), _p(mx), _p(mom), _p(mom_hi),
         int(threads or os.cpu_count() or 1))
     if rc:
         raise ValueError('oracle: an auxiliary side has no nodes')
-    return {'edge': edge, 'has_spanning_cluster': span.astype(bool),
-            'max_cluster_size': mx, 'moments': mom}
+    out = {'edge': edge, 'has_spanning_cluster': span.astype(bool),
+           'max_cluster_size': mx, 'moments': mom}
+    if exact:
+        out['moments_hi'] = mom_hi
+    return out
 
 
-def microcanonical_sums(traces):
+def exact_moments(traces):
+    """(R, S, 5) object array of the exact moments of bond_runs(exact=True)."""
+    return (traces['moments_hi'].astype(object) * (1 << 64) +
+            traces['moments'].astype(object))
+
+
+def microcanonical_sums(traces, limbs=False):
     """Exact sums over runs per step: (sum1, sum2) object arrays (7, S) of
-    Python ints, columns max, spanning, moments 0..4."""
+    Python ints, columns max, spanning, moments 0..4 (with limbs=True the
+    uint64 limb arrays (7, S, 2) and (7, S, 3) instead).  The moments are the
+    traces' exact values when they carry moments_hi (bond_runs(exact=True)),
+    else their 64-bit values."""
     lib = _load()
     span = np.ascontiguousarray(traces['has_spanning_cluster'], dtype=np.uint8)
     mx = np.ascontiguousarray(traces['max_cluster_size'], dtype=np.uint32)
     mom = np.ascontiguousarray(traces['moments'], dtype=np.uint64)
+    hi = traces.get('moments_hi')
+    hi = None if hi is None else np.ascontiguousarray(hi, dtype=np.uint64)
     R, S = mx.shape
     s1 = np.zeros((7, S, 2), dtype=np.uint64)
     s2 = np.zeros((7, S, 3), dtype=np.uint64)
-    lib.nzo_microcanonical_sums(R, S, _p(span), _p(mx), _p(mom), _p(s1), _p(s2))
+    if lib.nzo_microcanonical_sums(R, S, _p(span), _p(mx), _p(mom), _p(hi),
+                                   _p(s1), _p(s2)):
+        raise OverflowError('oracle: a sum exceeds its 128/192 bits')
+    if limbs:
+        return s1, s2
     return limbs_to_int(s1), limbs_to_int(s2)
+
+
+def bond_sums(g, perms, sums=None, threads=None):
+    """Exact per-step sums over the runs of perms (R, M), straight from the
+    sweep with exact moments (no traces kept): (sum1, sum2) uint64 limb
+    arrays (7, S, 2) and (7, S, 3).  With sums=(sum1, sum2) the new runs'
+    sums are added to those arrays in place (chunks of a large sample)."""
+    lib = _load()
+    M = g['num_edges']
+    S = M + 1
+    perms = np.ascontiguousarray(perms, dtype=np.int32).reshape(-1, M)
+    if sums is None:
+        s1 = np.zeros((7, S, 2), dtype=np.uint64)
+        s2 = np.zeros((7, S, 3), dtype=np.uint64)
+        acc = 0
+    else:
+        s1, s2 = sums
+        acc = 1
+    edges = np.ascontiguousarray(g['edges'], dtype=np.int64)
+    aux_side = np.ascontiguousarray(g['aux_side'], dtype=np.int32)
+    aux_edges = np.ascontiguousarray(g['aux_edges'], dtype=np.int64)
+    aux_edge_side = np.ascontiguousarray(g['aux_edge_side'], dtype=np.int32)
+    rc = lib.nzo_bond_sums(
+        perms.shape[0], g['num_nodes'], M, _p(edges), int(g['spanning']),
+        aux_side.size, _p(aux_side), aux_edge_side.size, _p(aux_edges),
+        _p(aux_edge_side), _p(perms), _p(s1), _p(s2), acc,
+        int(threads or os.cpu_count() or 1))
+    if rc == -1:
+        raise ValueError('oracle: an auxiliary side has no nodes')
+    if rc == -2:
+        raise OverflowError('oracle: a sum exceeds its 128/192 bits')
+    if rc:
+        raise MemoryError('oracle: out of memory')
+    return s1, s2
 
 
 def limbs_to_int(limbs):
