@@ -259,6 +259,28 @@ int pc_canonical_fused(uint32_t num_edges, uint32_t cols, const double *d_x,
                        double *d_out, int32_t precision, void *d_workspace,
                        uint64_t workspace_bytes, void *stream);
 
+/* Scratch bytes per GPU of pc_canonical_fused_noalloc (32 MiB). */
+#define PC_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/*
+ * pc_canonical_fused without a workspace argument and without allocating:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums go to a fixed scratch array of
+ * PC_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the
+ * process ends; a problem whose pc_canonical_fused_workspace_size exceeds it
+ * fails with PC_ENOMEM (every problem of at most 4096 p and 64 columns fits;
+ * at 21 columns, up to 63550 p).  Results are
+ * bit-identical to pc_canonical_fused.  Every call on one GPU uses that GPU's
+ * one array: calls that overlap on one GPU (other streams or host threads)
+ * race and must be serialised by the caller.
+ */
+int pc_canonical_fused_noalloc(uint32_t num_edges, uint32_t cols,
+                               const double *d_x, int64_t x_stride,
+                               uint32_t num_p, const double *d_ps,
+                               const int64_t *d_start, const int64_t *d_len,
+                               double *d_out, int32_t precision, void *stream);
+
 /* FP64 -> FP32 conversion of n elements (inputs of the FP32 precisions). */
 int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
                           void *stream);

@@ -30,6 +30,7 @@ PC_SWEEP_WAVES_MASK = 0x30
 PC_CHUNK = 64
 PC_STATE_WORDS = 8
 PC_COLUMNS = 7
+PC_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
@@ -69,6 +70,9 @@ SIGNATURES = {
                                                          ctypes.POINTER(_u64)]),
     'pc_canonical_fused': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32, _vp, _vp,
                                           _vp, _vp, _i32, _vp, _u64, _vp]),
+    'pc_canonical_fused_noalloc': (ctypes.c_int, [_u32, _u32, _vp, _i64, _u32,
+                                                  _vp, _vp, _vp, _vp, _i32,
+                                                  _vp]),
     'pc_microcanonical_ci': (ctypes.c_int, [_u32, _u32, _vp, _vp,
                                             ctypes.c_double, ctypes.c_double,
                                             _vp, _vp, _vp]),

@@ -29,6 +29,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "percolate_hip.h"
@@ -3029,32 +3030,31 @@ extern "C" int pc_canonical_fused_workspace_size(uint32_t num_p, uint32_t cols,
     return PC_OK;
 }
 
-extern "C" int pc_canonical_fused(uint32_t num_edges, uint32_t cols,
-                                  const double *d_x, int64_t x_stride,
-                                  uint32_t num_p, const double *d_ps,
-                                  const int64_t *d_start, const int64_t *d_len,
-                                  double *d_out, int32_t precision,
-                                  void *d_workspace, uint64_t workspace_bytes,
-                                  void *stream)
+// Checks of pc_canonical_fused's arguments; returns PC_OK or a failure.
+static int fused_check(const char *who, uint32_t cols, const double *d_x,
+                       int64_t x_stride, const double *d_ps,
+                       const int64_t *d_start, const int64_t *d_len,
+                       double *d_out)
 {
-    g_last_error[0] = 0;
-    if (num_p == 0 || cols == 0)
-        return PC_OK;
     if (cols > 4096u)
-        return pc_fail(PC_EOVERFLOW, "pc_canonical_fused: at most 4096 columns");
+        return pc_fail(PC_EOVERFLOW, "%s: at most 4096 columns", who);
     if (!d_x || !d_ps || !d_start || !d_len || !d_out || x_stride < (int64_t)cols)
-        return pc_fail(PC_EINVAL, "pc_canonical_fused: bad buffer");
-    uint64_t need = 0;
-    if (pc_canonical_fused_workspace_size(num_p, cols, precision, &need) != PC_OK)
-        return PC_EINVAL;
-    if (!d_workspace || workspace_bytes < need)
-        return pc_fail(PC_ENOMEM, "pc_canonical_fused: workspace needs %llu bytes",
-                       (unsigned long long)need);
-    hipStream_t s = as_stream(stream);
+        return pc_fail(PC_EINVAL, "%s: bad buffer", who);
+    return PC_OK;
+}
+
+// The launches of pc_canonical_fused with the partial sums in `partial`
+// (pc_canonical_fused_workspace_size bytes; every entry is written before
+// it is read, so it needs no clearing).
+static int fused_launch(uint32_t num_edges, uint32_t cols, const double *d_x,
+                        int64_t x_stride, uint32_t num_p, const double *d_ps,
+                        const int64_t *d_start, const int64_t *d_len,
+                        double *d_out, int32_t precision, double *partial,
+                        hipStream_t s)
+{
     const FusedCtx f{d_x, x_stride, d_ps, d_start, d_len, num_edges, num_p, cols};
     const uint32_t slices = contract_slices(num_p, cols + 1u);
     const dim3 grid((num_p + 15u) / 16u, (cols + 1u + 15u) / 16u, slices);
-    double *partial = (double *)d_workspace;
     if (precision == PC_PREC_FP64_MFMA) {
         canonical_fused_f64_kernel<<<grid, 64, 0, s>>>(f, slices, partial);
         PC_LAUNCH_CHECK("canonical_fused_f64_kernel");
@@ -3067,6 +3067,97 @@ extern "C" int pc_canonical_fused(uint32_t num_edges, uint32_t cols,
         partial, slices, num_p, cols, d_out);
     PC_LAUNCH_CHECK("fused_reduce_kernel");
     return PC_OK;
+}
+
+extern "C" int pc_canonical_fused(uint32_t num_edges, uint32_t cols,
+                                  const double *d_x, int64_t x_stride,
+                                  uint32_t num_p, const double *d_ps,
+                                  const int64_t *d_start, const int64_t *d_len,
+                                  double *d_out, int32_t precision,
+                                  void *d_workspace, uint64_t workspace_bytes,
+                                  void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0 || cols == 0)
+        return PC_OK;
+    int rc = fused_check("pc_canonical_fused", cols, d_x, x_stride, d_ps,
+                         d_start, d_len, d_out);
+    if (rc != PC_OK)
+        return rc;
+    uint64_t need = 0;
+    if (pc_canonical_fused_workspace_size(num_p, cols, precision, &need) != PC_OK)
+        return PC_EINVAL;
+    if (!d_workspace || workspace_bytes < need)
+        return pc_fail(PC_ENOMEM, "pc_canonical_fused: workspace needs %llu bytes",
+                       (unsigned long long)need);
+    return fused_launch(num_edges, cols, d_x, x_stride, num_p, d_ps, d_start,
+                        d_len, d_out, precision, (double *)d_workspace,
+                        as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of pc_canonical_fused_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address of each GPU's copy comes from hipGetSymbolAddress, looked
+// up once per device with that device current, and is used everywhere.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PC_NOALLOC_SCRATCH_BYTES];
+
+#define PC_MAX_DEVICES 64
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[PC_MAX_DEVICES];
+
+// Device address of the current device's scratch copy.
+static int scratch_address(void **out)
+{
+    int dev = -1;
+    PC_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= PC_MAX_DEVICES)
+        return pc_fail(PC_EINVAL, "scratch: device ordinal %d outside 0..%d", dev,
+                       PC_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        PC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return pc_fail(PC_EHIP, "scratch: hipGetSymbolAddress gave NULL");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PC_OK;
+}
+
+extern "C" int pc_canonical_fused_noalloc(uint32_t num_edges, uint32_t cols,
+                                          const double *d_x, int64_t x_stride,
+                                          uint32_t num_p, const double *d_ps,
+                                          const int64_t *d_start,
+                                          const int64_t *d_len, double *d_out,
+                                          int32_t precision, void *stream)
+{
+    g_last_error[0] = 0;
+    if (num_p == 0 || cols == 0)
+        return PC_OK;
+    int rc = fused_check("pc_canonical_fused_noalloc", cols, d_x, x_stride, d_ps,
+                         d_start, d_len, d_out);
+    if (rc != PC_OK)
+        return rc;
+    uint64_t need = 0;
+    if (pc_canonical_fused_workspace_size(num_p, cols, precision, &need) != PC_OK)
+        return PC_EINVAL;
+    if (need > (uint64_t)PC_NOALLOC_SCRATCH_BYTES)
+        return pc_fail(PC_ENOMEM,
+                       "pc_canonical_fused_noalloc: %u p x %u columns need %llu "
+                       "bytes of scratch, above the fixed %u (use "
+                       "pc_canonical_fused with a workspace)",
+                       num_p, cols, (unsigned long long)need,
+                       (unsigned)PC_NOALLOC_SCRATCH_BYTES);
+    void *p = nullptr;
+    rc = scratch_address(&p);
+    if (rc != PC_OK)
+        return rc;
+    return fused_launch(num_edges, cols, d_x, x_stride, num_p, d_ps, d_start,
+                        d_len, d_out, precision, (double *)p, as_stream(stream));
 }
 
 extern "C" int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
