@@ -56,17 +56,26 @@ int pc_record_bytes(uint32_t num_nodes);
 
 /*
  * Device permutations (performance mode).  Fills d_perm[r * perm_stride + i]
- * for r < num_runs, i < num_edges with a pseudo-random permutation of
- * 0..num_edges-1 for run first_run + r: a cycle-walking Feistel network on
- * ceil(log2 num_edges) bits (halves of floor and ceil of half the bits, each
- * round xoring a table lookup of one half into the other) whose eight round
- * functions are tables drawn from Philox4x32-10 keyed by seed.
- * Replaces the host-side numpy RandomState(seed).permutation(num_edges) of
+ * for r < num_runs, i < num_edges with a uniform random permutation of
+ * 0..num_edges-1 for run first_run + r, keyed by (seed, run): a
+ * Rao-Sandelius shuffle on Philox4x32-10 words (elements grouped into
+ * buckets of about 4096 by random labels in a stable counting sort, each
+ * bucket shuffled by a second level of labels and Fisher-Yates with
+ * unbiased bounded draws; exactly uniform given uniform words).  Replaces
+ * the host-side numpy RandomState(seed).permutation(num_edges) of
  * percolate/hpc.py:195,206 when bit-exactness with numpy is not required.
+ * d_workspace holds pc_permute_philox_workspace_size bytes (per-run bucket
+ * counts).  num_edges <= 2^23 and first_run + num_runs <= 2^32, else
+ * PC_EOVERFLOW.
  */
 int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
                       uint32_t num_edges, int32_t *d_perm, int64_t perm_stride,
+                      void *d_workspace, uint64_t workspace_bytes,
                       void *stream);
+
+/* Device workspace bytes pc_permute_philox needs. */
+int pc_permute_philox_workspace_size(uint32_t num_edges, uint32_t num_runs,
+                                     uint64_t *bytes);
 
 /* Where pc_sweep keeps the union-find state of a run. */
 #define PC_STATE_AUTO 0 /* LDS when num_nodes <= 16384, else HBM             */

@@ -21,8 +21,9 @@ step, citing the lines it follows:
 * ``microcanonical_step_stats`` percolate/percolate.py:450-705, 862-891
 * ``microcanonical_arrays``     percolate/percolate.py:968-1064
 * ``canonical_averages``        percolate/percolate.py:1112-1223
-* ``philox4x32_10`` and ``feistel_permutation``: the device-permutation
-  algorithm of this package (no reference counterpart), restated in Python.
+* ``philox4x32_10``, ``bucket_permutations`` (C, nz_shuffle.c) and
+  ``bucket_permutation_py``: the device-permutation algorithm of this package
+  (no reference counterpart), a Philox-keyed Rao-Sandelius shuffle.
 
 Pinning (tests/test_oracle_*.py): the sweep against the reference's own
 known-answer tests, the tutorial traces, brute-force connected components
@@ -70,6 +71,12 @@ def _load():
         lib.nzo_microcanonical_sums.restype = ctypes.c_int
         lib.nzo_microcanonical_sums.argtypes = [i64, i64, vp, vp, vp, vp, vp,
                                                 vp]
+        lib.nzo_philox4x32_10.restype = None
+        lib.nzo_philox4x32_10.argtypes = [vp, vp, vp]
+        lib.nzo_bucket_permutations.restype = ctypes.c_int
+        lib.nzo_bucket_permutations.argtypes = [ctypes.c_uint64, ctypes.c_uint32,
+                                                i64, i64, i64, i64, vp,
+                                                ctypes.c_int]
         lib.nzo_bond_sums.restype = ctypes.c_int
         lib.nzo_bond_sums.argtypes = [i64, i64, i64, vp, ctypes.c_int, i64,
                                       vp, i64, vp, vp, vp, vp, vp,
@@ -473,43 +480,118 @@ def philox4x32_10(ctr, key):
     return c0, c1, c2, c3
 
 
-PERM_ROUNDS = 8
-PERM_TAG = 0x7065726d
+# The size parameters of the device's layout (pc_permute_philox).
+SHUFFLE_BUCKET = 4096
+SHUFFLE_GROUP_CAP = 8192
 
 
-def feistel_permutation(seed, run, num_edges):
-    """The permutation pc_permute_philox draws for one run (Python ints):
-    an alternating Feistel network on b = ceil(log2 M) bits (high half of
-    floor(b/2) bits, low half of the rest), cycle-walked into [0, M)."""
+def philox4x32_10_c(ctr, key):
+    """The C restatement's Philox (nz_shuffle.c), for pinning it against the
+    known-answer vectors like philox4x32_10."""
+    lib = _load()
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib.nzo_philox4x32_10(_p(c), _p(k), _p(out))
+    return tuple(int(v) for v in out)
+
+
+def bucket_permutations(seed, first_run, runs, num_edges,
+                        bucket=SHUFFLE_BUCKET, group_cap=SHUFFLE_GROUP_CAP,
+                        threads=None):
+    """The permutations pc_permute_philox draws for runs first_run ..
+    first_run + runs - 1 (int32 array (runs, M)), restated sequentially in
+    C (nz_shuffle.c): a Philox-keyed Rao-Sandelius shuffle.  The size
+    parameters default to the device's; smaller ones put small inputs
+    through many buckets and the wide Fisher-Yates path."""
+    lib = _load()
     M = int(num_edges)
-    if M == 0:
-        return np.zeros(0, dtype=np.int64)
-    bits = 1
-    while (1 << bits) < M:
-        bits += 1
-    c = (bits + 1) // 2
-    a = bits - c
-    cmask, amask = (1 << c) - 1, (1 << a) - 1
-    entries = PERM_ROUNDS << c
-    table = np.zeros(entries, dtype=np.int64)
-    for g in range((entries + 7) // 8):
-        words = philox4x32_10((g, run & MASK32, run >> 32, PERM_TAG),
-                              (seed & MASK32, (seed >> 32) & MASK32))
-        for j in range(8):
-            e = g * 8 + j
-            if e < entries:
-                half = (words[j >> 1] >> (16 * (j & 1))) & 0xFFFF
-                table[e] = half & (cmask if (e >> c) & 1 else amask)
-    out = np.empty(M, dtype=np.int64)
-    for i in range(M):
-        y = i
-        while True:
-            L, R = y >> c, y & cmask
-            for j in range(0, PERM_ROUNDS, 2):
-                L ^= int(table[(j << c) | R])
-                R ^= int(table[((j + 1) << c) | L])
-            y = (L << c) | R
-            if y < M:
-                break
-        out[i] = y
+    out = np.zeros((int(runs), M), dtype=np.int32)
+    rc = lib.nzo_bucket_permutations(
+        int(seed) & 0xFFFFFFFFFFFFFFFF, int(first_run), int(runs), M,
+        int(bucket), int(group_cap), _p(out),
+        int(threads or os.cpu_count() or 1))
+    if rc:
+        raise ValueError('oracle: size outside the shuffle layout')
     return out
+
+
+def _half16(words, h):
+    return (words[h >> 1] >> (16 * (h & 1))) & 0xFFFF
+
+
+def bucket_permutation_py(seed, run, num_edges, bucket=SHUFFLE_BUCKET,
+                          group_cap=SHUFFLE_GROUP_CAP):
+    """The same layout as bucket_permutations, written again in pure Python
+    on top of philox4x32_10 (pinned by the Random123 vectors), for small
+    inputs: it pins the C restatement's counters, labels and draws."""
+    key = (seed & MASK32, (seed >> 32) & MASK32)
+
+    def key16(tag, b, j):
+        w = philox4x32_10((((j >> 9) << 6) | (j & 63), run, b, tag << 28), key)
+        return _half16(w, (j >> 6) & 7)
+
+    def stable_group(items, labels, nlab):
+        groups = [[] for _ in range(nlab)]
+        for it, lab in zip(items, labels):
+            groups[lab].append(it)
+        return groups
+
+    def shuffle_bucket(a, b):
+        n = len(a)
+        if n <= 1:
+            return list(a)
+        if n > group_cap:
+            a = list(a)
+            words, u = [], 0
+            for j in range(n - 1, 0, -1):
+                r = j + 1
+                while True:
+                    if not words:
+                        words = list(philox4x32_10((u, run, b, 4 << 28), key))
+                        u += 1
+                    m = words.pop(0) * r
+                    lo = m & MASK32
+                    if lo < r and lo < (2 ** 32 - r) % r:
+                        continue
+                    break
+                k = m >> 32
+                a[j], a[k] = a[k], a[j]
+            return a
+        lb = 0
+        while (32 << lb) < n and lb < 8:
+            lb += 1
+        labels = [key16(2, b, j) >> (16 - lb) if lb else 0 for j in range(n)]
+        groups = stable_group(a, labels, 1 << lb)
+        out, st = [], 0
+        for g in groups:
+            c = len(g)
+            halves, u = [], 0
+            for j in range(c - 1, 0, -1):
+                r = j + 1
+                while True:
+                    if not halves:
+                        w = philox4x32_10((st, run, b, (3 << 28) | u), key)
+                        halves = [_half16(w, h) for h in range(8)]
+                        u += 1
+                    m = halves.pop(0) * r
+                    lo = m & 0xFFFF
+                    if lo < r and lo < (65536 - r) % r:
+                        continue
+                    break
+                k = m >> 16
+                g[j], g[k] = g[k], g[j]
+            out.extend(g)
+            st += c
+        return out
+
+    M = int(num_edges)
+    lb1 = 0
+    while (bucket << lb1) < M:
+        lb1 += 1
+    labels = [key16(1, 0, i) >> (16 - lb1) if lb1 else 0 for i in range(M)]
+    buckets = stable_group(list(range(M)), labels, 1 << lb1)
+    out = []
+    for b, items in enumerate(buckets):
+        out.extend(shuffle_bucket(items, b))
+    return np.array(out, dtype=np.int64)

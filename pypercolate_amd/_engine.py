@@ -87,16 +87,31 @@ class BondEngine(object):
         perms = np.ascontiguousarray(perms, dtype=np.int32).reshape(-1, self.M)
         return self.torch.from_numpy(perms).to(self.device)
 
-    def device_permutations(self, seed, num_runs, first_run=0, out=None):
-        """Philox-keyed device permutations for runs first_run..+num_runs."""
+    def permutation_workspace(self, num_runs):
+        """Device workspace of device_permutations for num_runs runs."""
+        b = ctypes.c_uint64(0)
+        nat.check(self.lib.pc_permute_philox_workspace_size(
+            self.M, int(num_runs), ctypes.byref(b)),
+            'pc_permute_philox_workspace_size')
+        return (self.torch.empty((int(b.value),), dtype=self.torch.uint8,
+                                 device=self.device) if b.value else None)
+
+    def device_permutations(self, seed, num_runs, first_run=0, out=None,
+                            workspace=None):
+        """Philox-keyed uniform device permutations (pc_permute_philox) for
+        runs first_run..first_run + num_runs - 1."""
         torch = self.torch
         if out is None:
             out = torch.empty((num_runs, self.M), dtype=torch.int32,
                               device=self.device)
+        if workspace is None:
+            workspace = self.permutation_workspace(num_runs)
         st = nat.stream_handle(torch)
         nat.check(self.lib.pc_permute_philox(
             int(seed) & 0xFFFFFFFFFFFFFFFF, int(first_run), int(num_runs),
-            self.M, nat.ptr(out), self.M, st), 'pc_permute_philox')
+            self.M, nat.ptr(out), self.M, nat.ptr(workspace),
+            0 if workspace is None else int(workspace.numel()), st),
+            'pc_permute_philox')
         return out
 
     def upload_seeds(self, seeds):
@@ -290,7 +305,8 @@ class Pipeline(object):
     with seeds= (R integers) numpy RandomState(seeds[r]).permutation(M),
     bit-identical to the reference's draw, on the device."""
 
-    def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None):
+    def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None,
+                 permute_ahead=None):
         torch = self.torch = engine.torch
         self.engine = eng = engine
         self.R = int(num_runs)
@@ -318,6 +334,10 @@ class Pipeline(object):
                 # to hide its latencies; they, not the sweeps, set the pace
                 sub_batch *= 4
         self.sub = max(1, min(int(sub_batch), self.R))
+        # Philox permutations of every run in one launch before the sweeps
+        # (the shuffle needs 72 KiB of LDS per run, so beside the sweeps it
+        # waits for LDS they hold) or per sub-batch beside the sweeps
+        self.permute_ahead = (seeds is None) if permute_ahead is None else bool(permute_ahead)
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
         self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
         self.records = torch.empty((self.R, max(M, 1)), dtype=eng.rec_torch,
@@ -336,6 +356,9 @@ class Pipeline(object):
                            if wb else None for _ in range(min(2, nsub))]
         self.streams = _pipeline_streams(torch, dev)
         self.sweep_events = []
+        self.perm_ws = (eng.permutation_workspace(self.R if self.permute_ahead
+                                                  else self.sub)
+                        if seeds is None else None)
 
     def run(self):
         """Enqueue the whole batch; the caller's stream waits for it."""
@@ -351,17 +374,30 @@ class Pipeline(object):
         self.sweep_events = []
         first = True
         fused = eng.fuse_chunk_states()
+        if self.permute_ahead:
+            with torch.cuda.stream(sp):
+                if self.d_seeds is None:
+                    eng.device_permutations(self.seed, self.R, out=self.perm,
+                                            workspace=self.perm_ws)
+                else:
+                    eng.randomstate_permutations(self.d_seeds, out=self.perm)
+                have_all = torch.cuda.Event()
+                have_all.record(sp)
         for i, b0 in enumerate(range(0, self.R, self.sub)):
             n = min(self.sub, self.R - b0)
             ss = (ss0, ss1)[i & 1]
             perm = self.perm[b0:b0 + n]
-            with torch.cuda.stream(sp):
-                if self.d_seeds is None:
-                    eng.device_permutations(self.seed, n, first_run=b0, out=perm)
-                else:
-                    eng.randomstate_permutations(self.d_seeds[b0:b0 + n], out=perm)
-                have_perm = torch.cuda.Event()
-                have_perm.record(sp)
+            if self.permute_ahead:
+                have_perm = have_all
+            else:
+                with torch.cuda.stream(sp):
+                    if self.d_seeds is None:
+                        eng.device_permutations(self.seed, n, first_run=b0, out=perm,
+                                                workspace=self.perm_ws)
+                    else:
+                        eng.randomstate_permutations(self.d_seeds[b0:b0 + n], out=perm)
+                    have_perm = torch.cuda.Event()
+                    have_perm.record(sp)
             ss.wait_event(have_perm)
             with torch.cuda.stream(ss):
                 e0 = torch.cuda.Event(enable_timing=True)

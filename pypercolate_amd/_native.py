@@ -43,7 +43,10 @@ SIGNATURES = {
     'pc_abi_version': (ctypes.c_int, []),
     'pc_last_error': (ctypes.c_char_p, []),
     'pc_record_bytes': (ctypes.c_int, [_u32]),
-    'pc_permute_philox': (ctypes.c_int, [_u64, _u64, _u32, _u32, _vp, _i64, _vp]),
+    'pc_permute_philox': (ctypes.c_int, [_u64, _u64, _u32, _u32, _vp, _i64, _vp,
+                                         _u64, _vp]),
+    'pc_permute_philox_workspace_size': (ctypes.c_int, [_u32, _u32,
+                                                        ctypes.POINTER(_u64)]),
     'pc_permute_randomstate': (ctypes.c_int, [_vp, _u32, _u32, _vp, _i64, _vp]),
     'pc_sweep_workspace_size': (ctypes.c_int, [_u32, _u32, _u32,
                                                ctypes.POINTER(_u64)]),

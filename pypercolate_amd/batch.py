@@ -50,6 +50,7 @@ class _LengthPlan(object):
                                 device=dev)
         self.sum2 = torch.empty((nat.PC_COLUMNS, S, 3), dtype=torch.int64,
                                 device=dev)
+        self.perm_ws = eng.permutation_workspace(runs)
         wb = eng.workspace_bytes(runs)
         self.workspace = (torch.empty((wb,), dtype=torch.uint8, device=dev)
                           if wb else None)
@@ -63,7 +64,8 @@ class _LengthPlan(object):
     def launch(self):
         """Enqueue this L's pipeline on the current stream."""
         eng = self.engine
-        eng.device_permutations(self.seed, self.runs, out=self.perm)
+        eng.device_permutations(self.seed, self.runs, out=self.perm,
+                                workspace=self.perm_ws)
         fused = eng.fuse_chunk_states()
         res = eng.sweep(self.perm, records=self.records, span_step=self.span,
                         workspace=self.workspace,

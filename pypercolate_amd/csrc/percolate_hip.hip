@@ -3,7 +3,7 @@
 // include/percolate_hip.h.
 //
 // Pipeline for R runs on a graph with N nodes and M edges:
-//   permute (Philox-keyed Feistel, or MT19937 + numpy's shuffle for
+//   permute (Philox-keyed bucket shuffle, or MT19937 + numpy's shuffle for
 //   bit-exact RandomState permutations) -> sweep (union-find, one run per
 //   workgroup of one or two wavefronts, merge records) -> chunk states
 //   (per-run state every 64
@@ -189,8 +189,8 @@ __device__ __forceinline__ void atomic_add_u192(uint64_t *p, uint64_t w0,
 }
 
 // ---------------------------------------------------------------------------
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), used as the keyed PRF
-// that fills the Feistel round tables.
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based
+// generator of the device permutations.
 // ---------------------------------------------------------------------------
 struct u32x4 {
     uint32_t x, y, z, w;
@@ -214,82 +214,532 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
 }
 
 // ---------------------------------------------------------------------------
-// Device permutations: a Feistel network on b = ceil(log2 M) bits, split
-// into a high half L of a = floor(b / 2) bits and a low half R of
-// c = b - a bits, with PERM_ROUNDS random round tables drawn per run from
-// Philox4x32-10(counter = (table word, run lo, run hi, tag), key = seed).
-// Even rounds j set L ^= T_j[R] (a-bit values), odd rounds R ^= T_j[L]
-// (c-bit values): each round is invertible, so the network is a bijection
-// of [0, 2^b), and cycle walking restricts it to [0, M).  (For even b this
-// equals the swapping form (L, R) -> (R, L ^ T_j[R]) after any even number
-// of rounds; an odd b no longer needs a domain twice as large.)
-// perm[i] = edge index added at step i + 1.
+// Device permutations (performance mode): a Rao-Sandelius shuffle keyed by
+// Philox4x32-10.  Every element draws a uniform random bucket label; the
+// elements are grouped by label in a stable counting sort (one radix pass
+// over random keys), and every bucket is shuffled uniformly: a second level
+// of random labels, then Fisher-Yates with unbiased bounded draws inside
+// each sub-bucket.  With independent uniform words the result is an exactly
+// uniform permutation: iid labels plus a uniform order inside each label
+// class is a sort by iid continuous keys.
+//
+// Layout (the oracle's nz_shuffle.c restates it; perm[i] = edge index added
+// at step i + 1):
+//   - 2^lb1 buckets, lb1 the least with M <= SHUF_BUCKET << lb1; label of
+//     element i = the top lb1 bits of key(1, 0, i) (none when lb1 = 0).
+//   - Bucket b of n elements, in their grouped (increasing) order: n >
+//     SHUF_GROUP_CAP (odds below e^-1500 at these sizes): plain Fisher-Yates
+//     in HBM with 32-bit draws from ctr (u, run, b, 4 << 28).  Otherwise
+//     2^lb2 sub-buckets (the least lb2 <= 8 with n <= 32 << lb2), label of
+//     position j = the top lb2 bits of key(2, b, j), stable grouping, then
+//     Fisher-Yates of each sub-bucket (start st, size c) with 16-bit draws
+//     from ctr (st, run, b, 3 << 28 | u).
+//   - key(tag, b, j) = 16-bit half (j >> 6) & 7 (low half first) of
+//     Philox(ctr ((j >> 9) << 6 | (j & 63), run, b, tag << 28)): one Philox
+//     call per lane labels 8 positions 64 apart, so a wave labels 512
+//     consecutive positions per call and visits them in order.
+//   - Bounded draw on [0, r) from a w-bit word x: m = x * r, rejected while
+//     (m mod 2^w) < (2^w - r) mod r (Lemire), result m >> w.
+// Kernels: count (labels per bucket and chunk of SHUF_CHUNK elements) ->
+// scan (per run) -> scatter (stable grouping into the output row) -> group
+// (one workgroup per bucket, in LDS).  Small workgroups with little LDS, so
+// many run per CU and hide the latencies of the sequential parts.
 // ---------------------------------------------------------------------------
-#define PERM_ROUNDS 8
-#define PERM_BLOCK 256
-#ifndef PERM_ITEMS
-#define PERM_ITEMS 16
-#endif
-#define PERM_TAG 0x7065726du
+#define SHUF_BUCKET 4096u
+#define SHUF_GROUP_CAP 8192u
+#define SHUF_CHUNK 4096u
+#define SHUF_MAX_EDGES (1u << 23)
+#define SHUF_W 4
+#define SHUF_GW 4             // waves of a bucket's workgroup
+#define SHUF_HIST_WORDS 128u  // per wave: 256 16-bit sub-bucket counters
+// elements each thread of a bucket's workgroup gathers at most, and groups
+// of 512 positions each of its waves labels at most
+#define SHUF_GATHER (SHUF_GROUP_CAP / (64u * SHUF_GW))
+#define SHUF_GROUPS_PER_WAVE ((SHUF_GROUP_CAP / 512u + SHUF_GW - 1u) / SHUF_GW)
 
-__global__ __launch_bounds__(PERM_BLOCK) void permute_kernel(
-    uint32_t seed_lo, uint32_t seed_hi, uint64_t first_run, uint32_t M,
-    uint32_t bits, int32_t *__restrict__ perm, int64_t stride)
+__device__ __forceinline__ uint32_t half16(const u32x4 &w, uint32_t h)
 {
-    // round j's table holds 2^c entries (odd rounds use the first 2^a)
-    extern __shared__ uint16_t table[];
-    const uint64_t run = first_run + blockIdx.y;
-    const uint32_t c = (bits + 1u) / 2u, a = bits - c;
-    const uint32_t entries = PERM_ROUNDS << c;
-    const uint32_t cmask = (1u << c) - 1u, amask = (1u << a) - 1u;
+    const uint32_t wd = (h & 4u) ? ((h & 2u) ? w.w : w.z) : ((h & 2u) ? w.y : w.x);
+    return (wd >> ((h & 1u) * 16u)) & 0xFFFFu;
+}
 
-    for (uint32_t g = threadIdx.x; g * 8u < entries; g += PERM_BLOCK) {
-        const u32x4 o = philox4x32_10(
-            u32x4{g, (uint32_t)run, (uint32_t)(run >> 32), PERM_TAG}, seed_lo,
-            seed_hi);
-        const uint32_t words[4] = {o.x, o.y, o.z, o.w};
+__device__ __forceinline__ u32x4 shuf_philox(uint32_t x, uint32_t run, uint32_t b,
+                                             uint32_t w, uint32_t k0, uint32_t k1)
+{
+    return philox4x32_10(u32x4{x, run, b, w}, k0, k1);
+}
+
+// Lanes (of the active ones in `valid`) whose label equals this lane's, for a
+// label of `bits` bits: one ballot per bit.
+__device__ __forceinline__ uint64_t match_label(uint32_t label, uint32_t bits,
+                                                uint64_t valid)
+{
+    uint64_t m = valid;
+    for (uint32_t i = 0; i < bits; ++i) {
+        const bool set = (label >> i) & 1u;
+        const uint64_t ones = __ballot(set);
+        m &= set ? ones : ~ones;
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    return (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Lemire's rejection threshold (2^w - r) mod r for w = 16 and w = 32, called
+// only when a draw's low part is below r.  The empty asm hides r from the
+// optimiser, which would otherwise hoist the integer division out of the
+// rare branch and pay it on every draw.
+__device__ __forceinline__ uint32_t lemire_threshold16(uint32_t r)
+{
+    asm volatile("" : "+v"(r));
+    return (65536u - r) % r;
+}
+__device__ __forceinline__ uint32_t lemire_threshold32(uint32_t r)
+{
+    asm volatile("" : "+v"(r));
+    return (0u - r) % r;
+}
+
+// Fisher-Yates of a[0..c) with the 16-bit draw stream of sub-bucket start st.
+// The Philox words are kept as scalars: a struct indexed by the running half
+// counter would be demoted to a private (scratch or LDS) array.
+template <typename E>
+__device__ __forceinline__ void shuffle_sub(E *a, uint32_t c, uint32_t st,
+                                            uint32_t run, uint32_t b, uint32_t k0,
+                                            uint32_t k1)
+{
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    uint32_t h = 8u, u = 0u;
+    for (uint32_t j = c - 1u; j >= 1u; --j) {
+        const uint32_t r = j + 1u;
+        uint32_t m;
+        for (;;) {
+            if (h == 8u) {
+                const u32x4 w = shuf_philox(st, run, b, (3u << 28) | u, k0, k1);
+                w0 = w.x;
+                w1 = w.y;
+                w2 = w.z;
+                w3 = w.w;
+                ++u;
+                h = 0u;
+            }
+            const uint32_t wd = (h & 4u) ? ((h & 2u) ? w3 : w2) : ((h & 2u) ? w1 : w0);
+            m = ((wd >> ((h & 1u) * 16u)) & 0xFFFFu) * r;
+            ++h;
+            const uint32_t lo = m & 0xFFFFu;
+            if (__builtin_expect(lo < r, 0) && lo < lemire_threshold16(r))
+                continue;
+            break;
+        }
+        const uint32_t k = m >> 16;
+        const E t = a[j];
+        a[j] = a[k];
+        a[k] = t;
+    }
+}
+
+// Fisher-Yates of a bucket of n > SHUF_GROUP_CAP elements in HBM, 32-bit
+// draws (one lane).
+__device__ __noinline__ void shuffle_wide(int32_t *a, uint32_t n, uint32_t run,
+                                          uint32_t b, uint32_t k0, uint32_t k1)
+{
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    uint32_t h = 4u, u = 0u;
+    for (uint32_t j = n - 1u; j >= 1u; --j) {
+        const uint32_t r = j + 1u;
+        uint64_t m;
+        for (;;) {
+            if (h == 4u) {
+                const u32x4 w = shuf_philox(u, run, b, 4u << 28, k0, k1);
+                w0 = w.x;
+                w1 = w.y;
+                w2 = w.z;
+                w3 = w.w;
+                ++u;
+                h = 0u;
+            }
+            const uint32_t x = h == 0u ? w0 : h == 1u ? w1 : h == 2u ? w2 : w3;
+            ++h;
+            m = (uint64_t)x * r;
+            const uint32_t lo = (uint32_t)m;
+            if (__builtin_expect(lo < r, 0) && lo < lemire_threshold32(r))
+                continue;
+            break;
+        }
+        const uint32_t k = (uint32_t)(m >> 32);
+        const int32_t t = a[j];
+        a[j] = a[k];
+        a[k] = t;
+    }
+}
+
+// Bucket labels of a chunk of elements: the wave's groups of 512 positions
+// within the chunk, and its range.
+struct ShufChunk {
+    uint32_t g0, g1;  // the wave's group range (global group indices)
+    uint32_t lo, hi;  // the chunk's element range
+};
+
+__device__ __forceinline__ ShufChunk shuf_chunk(uint32_t c, uint32_t M)
+{
+    const uint32_t wave = threadIdx.x >> 6;
+    ShufChunk k;
+    k.lo = c * SHUF_CHUNK;
+    k.hi = min(M, k.lo + SHUF_CHUNK);
+    const uint32_t first = k.lo / 512u, G = (k.hi - k.lo + 511u) / 512u;
+    k.g0 = first + wave * G / SHUF_W;
+    k.g1 = first + (wave + 1u) * G / SHUF_W;
+    return k;
+}
+
+// Count kernel: counts[(r * nb1 + b) * chunks + c] = elements of chunk c of
+// run r with bucket label b.
+__global__ __launch_bounds__(64 * SHUF_W) void shuffle_count_kernel(
+    uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t M, uint32_t lb1,
+    uint32_t chunks, uint32_t r0, uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t cnt[];  // 2^lb1
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, nb1 = 1u << lb1;
+    const uint32_t c = blockIdx.x, r = r0 + blockIdx.y, run = first_run + r;
+    for (uint32_t i = tid; i < nb1; i += 64u * SHUF_W)
+        cnt[i] = 0u;
+    __syncthreads();
+    const ShufChunk ch = shuf_chunk(c, M);
+    for (uint32_t g = ch.g0; g < ch.g1; ++g) {
+        const u32x4 key = lb1 ? shuf_philox((g << 6) | lane, run, 0u, 1u << 28, k0, k1)
+                              : u32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t e = g * 8u + (uint32_t)j;
-            if (e < entries)
-                table[e] = (uint16_t)((words[j >> 1] >> (16 * (j & 1))) &
-                                      (((e >> c) & 1u) ? cmask : amask));
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t i = g * 512u + k * 64u + lane;
+            if (i < ch.hi)
+                atomicAdd(&cnt[lb1 ? half16(key, k) >> (16u - lb1) : 0u], 1u);
         }
     }
     __syncthreads();
+    for (uint32_t b = tid; b < nb1; b += 64u * SHUF_W)
+        counts[((uint64_t)r * nb1 + b) * chunks + c] = cnt[b];
+}
 
-    int32_t *out = perm + (int64_t)blockIdx.y * stride;
-    const uint32_t base = blockIdx.x * (PERM_BLOCK * PERM_ITEMS);
-    // Each lane maps its PERM_ITEMS elements in turn, one Feistel pass per
-    // trip: an image outside [0, M) is mapped again (cycle walking), an
-    // image inside is stored and the lane moves on to its next element.  A
-    // pass lands inside with probability M / 2^b, as low as 1/2, so a
-    // loop that waited for every lane of the wave before taking the next
-    // element would run about log2(64) + 1 passes per element instead of
-    // about two; here a lane that finished starts its next element while
-    // the others walk.
-    uint32_t k = 0;
-    uint32_t i = base + threadIdx.x;
-    uint32_t y = i;
-    bool active = i < M;
-    while (active) {
-        uint32_t L = y >> c, R = y & cmask;
-#pragma unroll
-        for (uint32_t j = 0; j < PERM_ROUNDS; j += 2u) {
-            L ^= table[(j << c) | R];
-            R ^= table[((j + 1u) << c) | L];
+// Scan kernel, one workgroup per run: counts become exclusive prefixes in
+// (bucket major, chunk minor) order, starts[r * (nb1 + 1) + b] the first
+// position of bucket b (starts[.. + nb1] = M).  Each wave reads whole
+// buckets' chunk counts, 64 at a time (coalesced); bucket totals are
+// scanned in LDS in between.
+__global__ __launch_bounds__(256) void shuffle_scan_kernel(
+    uint32_t M, uint32_t lb1, uint32_t chunks, uint32_t r0,
+    uint32_t *__restrict__ counts, uint32_t *__restrict__ starts)
+{
+    extern __shared__ uint32_t tot[];  // 2^lb1 bucket totals, then bases
+    __shared__ uint32_t wsum[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t r = r0 + blockIdx.x, nb1 = 1u << lb1;
+    uint32_t *q = counts + (uint64_t)r * nb1 * chunks;
+    uint32_t *st = starts + (uint64_t)r * (nb1 + 1u);
+    for (uint32_t b = wave; b < nb1; b += 4u) {
+        uint32_t acc = 0;
+        for (uint32_t c = lane; c < chunks; c += 64u)
+            acc += q[(uint64_t)b * chunks + c];
+        acc = wave_scan_add(acc);
+        if (lane == 63u)
+            tot[b] = acc;
+    }
+    __syncthreads();
+    {
+        const uint32_t per = (nb1 + 255u) / 256u;
+        const uint32_t a = min(tid * per, nb1), e = min(a + per, nb1);
+        uint32_t sum = 0;
+        for (uint32_t b = a; b < e; ++b)
+            sum += tot[b];
+        const uint32_t inc = wave_scan_add(sum);
+        if (lane == 63u)
+            wsum[wave] = inc;
+        __syncthreads();
+        uint32_t acc = inc - sum;
+        for (uint32_t w = 0; w < wave; ++w)
+            acc += wsum[w];
+        for (uint32_t b = a; b < e; ++b) {
+            const uint32_t t = tot[b];
+            tot[b] = acc;
+            st[b] = acc;
+            acc += t;
         }
-        y = (L << c) | R;
-        if (y < M) {
-            out[i] = (int32_t)y;
-            ++k;
-            i = base + k * PERM_BLOCK + threadIdx.x;
-            y = i;
-            active = k < (uint32_t)PERM_ITEMS && i < M;
+        if (tid == 0u)
+            st[nb1] = M;
+    }
+    __syncthreads();
+    for (uint32_t b = wave; b < nb1; b += 4u) {
+        uint32_t carry = tot[b];
+        for (uint32_t c0 = 0; c0 < chunks; c0 += 64u) {
+            const uint32_t c = c0 + lane;
+            const uint32_t v = c < chunks ? q[(uint64_t)b * chunks + c] : 0u;
+            const uint32_t inc = wave_scan_add(v);
+            if (c < chunks)
+                q[(uint64_t)b * chunks + c] = carry + inc - v;
+            carry += read_lane(inc, 63u);
         }
     }
 }
 
+// Scatter kernel: the elements of chunk c of run r, sorted by bucket label
+// in LDS (stable), then written bucket by bucket with consecutive threads on
+// consecutive addresses (whole cache lines, instead of one scattered 4-byte
+// store per element).
+__global__ __launch_bounds__(64 * SHUF_W) void shuffle_scatter_kernel(
+    uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t M, uint32_t lb1,
+    uint32_t chunks, uint32_t r0, const uint32_t *__restrict__ counts,
+    int32_t *__restrict__ perm, int64_t stride)
+{
+    __shared__ int32_t sval[SHUF_CHUNK];
+    __shared__ uint16_t slab[SHUF_CHUNK];
+    __shared__ uint32_t wsum[SHUF_W];
+    extern __shared__ uint32_t dyn[];  // [SHUF_W][nb1] bases, lstart, gbase
+    const uint32_t T = 64u * SHUF_W, tid = threadIdx.x, lane = tid & 63u,
+                   wave = tid >> 6;
+    const uint32_t nb1 = 1u << lb1;
+    const uint32_t c = blockIdx.x, r = r0 + blockIdx.y, run = first_run + r;
+    uint32_t *base = dyn, *lstart = dyn + SHUF_W * nb1, *gbase = lstart + nb1;
+    uint32_t *my = base + wave * nb1;
+    for (uint32_t i = tid; i < SHUF_W * nb1; i += T)
+        base[i] = 0u;
+    __syncthreads();
+    const ShufChunk ch = shuf_chunk(c, M);
+    for (uint32_t g = ch.g0; g < ch.g1; ++g) {
+        const u32x4 key = lb1 ? shuf_philox((g << 6) | lane, run, 0u, 1u << 28, k0, k1)
+                              : u32x4{0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t i = g * 512u + k * 64u + lane;
+            if (i < ch.hi)
+                atomicAdd(&my[lb1 ? half16(key, k) >> (16u - lb1) : 0u], 1u);
+        }
+    }
+    __syncthreads();
+    // local bucket starts (exclusive over buckets of the chunk's totals),
+    // per-wave local bases, and the chunk's global base of each bucket
+    {
+        const uint32_t per = (nb1 + T - 1u) / T;
+        const uint32_t a = min(tid * per, nb1), e = min(a + per, nb1);
+        uint32_t sum = 0;
+        for (uint32_t b = a; b < e; ++b)
+            for (uint32_t w = 0; w < (uint32_t)SHUF_W; ++w)
+                sum += base[w * nb1 + b];
+        const uint32_t inc = wave_scan_add(sum);
+        if (lane == 63u)
+            wsum[wave] = inc;
+        __syncthreads();
+        uint32_t acc = inc - sum;
+        for (uint32_t w = 0; w < wave; ++w)
+            acc += wsum[w];
+        for (uint32_t b = a; b < e; ++b) {
+            lstart[b] = acc;
+            gbase[b] = counts[((uint64_t)r * nb1 + b) * chunks + c];
+            for (uint32_t w = 0; w < (uint32_t)SHUF_W; ++w) {
+                const uint32_t n = base[w * nb1 + b];
+                base[w * nb1 + b] = acc;
+                acc += n;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t g = ch.g0; g < ch.g1; ++g) {
+        const u32x4 key = lb1 ? shuf_philox((g << 6) | lane, run, 0u, 1u << 28, k0, k1)
+                              : u32x4{0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t i = g * 512u + k * 64u + lane;
+            const bool ok = i < ch.hi;
+            const uint64_t valid = __ballot(ok);
+            const uint32_t bkt = lb1 ? half16(key, k) >> (16u - lb1) : 0u;
+            const uint64_t peers = match_label(bkt, lb1, valid);
+            if (ok) {
+                const uint32_t rank = lanes_below(peers);
+                const uint32_t at = my[bkt];
+                sval[at + rank] = (int32_t)i;
+                slab[at + rank] = (uint16_t)bkt;
+                if (rank == 0u)
+                    my[bkt] = at + (uint32_t)__popcll(peers);
+            }
+        }
+    }
+    __syncthreads();
+    int32_t *row = perm + (int64_t)r * stride;
+    const uint32_t n = ch.hi - ch.lo;
+    for (uint32_t t = tid; t < n; t += T) {
+        const uint32_t b = slab[t];
+        row[gbase[b] + (t - lstart[b])] = sval[t];
+    }
+}
+
+// One bucket a[0..n), n <= SHUF_GROUP_CAP, shuffled by a workgroup of W
+// waves: sub-bucket labels, stable grouping of the local indices 0..n-1 in
+// ent[] (16-bit), Fisher-Yates of each sub-bucket, then a[t] = old a[ent[t]]
+// (all loads in flight together, all stores after a barrier).  hist holds
+// W * SHUF_HIST_WORDS words (16-bit counters in pairs), wsum W.
+template <int W>
+__device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
+                                  uint32_t k0, uint32_t k1, int32_t *a,
+                                  uint16_t *ent, uint32_t *hist, uint32_t *wsum)
+{
+    const uint32_t T = 64u * W, tid = threadIdx.x, wave = tid >> 6,
+                   lane = tid & 63u;
+    uint32_t lb = 0;
+    while ((32u << lb) < n && lb < 8u)
+        ++lb;
+    const uint32_t nb = 1u << lb;
+    const uint32_t G = (n + 511u) / 512u;
+    const uint32_t g0 = wave * G / W, g1 = (wave + 1u) * G / W;
+    uint32_t *my = hist + wave * SHUF_HIST_WORDS;
+
+    for (uint32_t i = tid; i < W * SHUF_HIST_WORDS; i += T)
+        hist[i] = 0u;
+    __syncthreads();
+
+    // labels of this wave's positions (8 per Philox call, kept for the
+    // grouping below: at most SHUF_GROUPS_PER_WAVE groups), counted per wave
+    u32x4 keys[SHUF_GROUPS_PER_WAVE];
+#pragma unroll
+    for (uint32_t gi = 0; gi < SHUF_GROUPS_PER_WAVE; ++gi) {
+        const uint32_t g = g0 + gi;
+        keys[gi] = (lb && g < g1)
+                       ? shuf_philox((g << 6) | lane, run, b, 2u << 28, k0, k1)
+                       : u32x4{0, 0, 0, 0};
+        if (g < g1) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                const uint32_t j = g * 512u + k * 64u + lane;
+                if (j < n) {
+                    const uint32_t s = lb ? half16(keys[gi], k) >> (16u - lb) : 0u;
+                    atomicAdd(&my[s >> 1], 1u << ((s & 1u) * 16u));
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // exclusive prefix over (sub-bucket major, wave minor); each thread owns
+    // whole words (both 16-bit counters of a sub-bucket pair, all waves)
+    {
+        const uint32_t pairs = (nb + 1u) / 2u;
+        const uint32_t per = (pairs + T - 1u) / T;
+        const uint32_t p0 = min(tid * per, pairs), p1 = min(p0 + per, pairs);
+        uint32_t tot = 0;
+        for (uint32_t p = p0; p < p1; ++p)
+            for (uint32_t w = 0; w < (uint32_t)W; ++w) {
+                const uint32_t word = hist[w * SHUF_HIST_WORDS + p];
+                tot += (word & 0xFFFFu) + (word >> 16);
+            }
+        const uint32_t inc = wave_scan_add(tot);
+        if (lane == 63u)
+            wsum[wave] = inc;
+        __syncthreads();
+        uint32_t run_ = inc - tot;
+        for (uint32_t w = 0; w < wave; ++w)
+            run_ += wsum[w];
+        for (uint32_t p = p0; p < p1; ++p) {
+            // sub-bucket 2p (low halves) of every wave, then 2p + 1 (high)
+            uint32_t lo_total = 0;
+            for (uint32_t w = 0; w < (uint32_t)W; ++w)
+                lo_total += hist[w * SHUF_HIST_WORDS + p] & 0xFFFFu;
+            uint32_t a0 = run_, a1 = run_ + lo_total;
+            for (uint32_t w = 0; w < (uint32_t)W; ++w) {
+                const uint32_t word = hist[w * SHUF_HIST_WORDS + p];
+                hist[w * SHUF_HIST_WORDS + p] = a0 | (a1 << 16);
+                a0 += word & 0xFFFFu;
+                a1 += word >> 16;
+            }
+            run_ = a1;
+        }
+    }
+    __syncthreads();
+
+    // stable grouping of the local indices: positions in increasing order,
+    // rank among the lanes of the same label, the label's first lane
+    // advances the wave's base
+#pragma unroll
+    for (uint32_t gi = 0; gi < SHUF_GROUPS_PER_WAVE; ++gi) {
+        const uint32_t g = g0 + gi;
+        if (g < g1) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                const uint32_t j = g * 512u + k * 64u + lane;
+                const bool ok = j < n;
+                const uint64_t valid = __ballot(ok);
+                const uint32_t s = lb ? half16(keys[gi], k) >> (16u - lb) : 0u;
+                const uint64_t peers = match_label(s, lb, valid);
+                if (ok) {
+                    const uint32_t rank = lanes_below(peers);
+                    const uint32_t sh = (s & 1u) * 16u;
+                    const uint32_t at = (my[s >> 1] >> sh) & 0xFFFFu;
+                    ent[at + rank] = (uint16_t)j;
+                    if (rank == 0u)
+                        atomicAdd(&my[s >> 1], (uint32_t)__popcll(peers) << sh);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // Fisher-Yates per sub-bucket: after the grouping the last wave's base
+    // of sub-bucket s is the end of s
+    const uint32_t *last = hist + (W - 1) * SHUF_HIST_WORDS;
+    for (uint32_t s = tid; s < nb; s += T) {
+        const uint32_t en = (last[s >> 1] >> ((s & 1u) * 16u)) & 0xFFFFu;
+        const uint32_t st =
+            s ? (last[(s - 1u) >> 1] >> (((s - 1u) & 1u) * 16u)) & 0xFFFFu : 0u;
+        if (en - st >= 2u)
+            shuffle_sub(ent + st, en - st, st, run, b, k0, k1);
+    }
+    __syncthreads();
+
+    // a[t] = old a[ent[t]]: every load issued before any store of the
+    // workgroup (the empty asm makes each thread wait for its loads before
+    // the barrier)
+    int32_t v[SHUF_GATHER];
+#pragma unroll
+    for (uint32_t i = 0; i < SHUF_GATHER; ++i) {
+        const uint32_t t = tid + i * T;
+        v[i] = t < n ? a[ent[t]] : 0;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < SHUF_GATHER; ++i)
+        asm volatile("" ::"v"(v[i]));
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < SHUF_GATHER; ++i) {
+        const uint32_t t = tid + i * T;
+        if (t < n)
+            a[t] = v[i];
+    }
+}
+
+// Group kernel: bucket b = blockIdx.x of run r0 + blockIdx.y, in place in
+// the output row.
+__global__ __launch_bounds__(64 * SHUF_GW) void shuffle_group_kernel(
+    uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t lb1, uint32_t r0,
+    const uint32_t *__restrict__ starts, int32_t *__restrict__ perm,
+    int64_t stride)
+{
+    __shared__ uint16_t ent[SHUF_GROUP_CAP];
+    __shared__ uint32_t hist[SHUF_GW * SHUF_HIST_WORDS];
+    __shared__ uint32_t wsum[SHUF_GW];
+    const uint32_t nb1 = 1u << lb1, b = blockIdx.x, r = r0 + blockIdx.y;
+    const uint32_t run = first_run + r;
+    const uint32_t *st = starts + (uint64_t)r * (nb1 + 1u);
+    const uint32_t o = st[b], n = st[b + 1u] - o;
+    int32_t *a = perm + (int64_t)r * stride + o;
+    if (n <= 1u)
+        return;
+    if (n > SHUF_GROUP_CAP) {
+        if (threadIdx.x == 0u)
+            shuffle_wide(a, n, run, b, k0, k1);
+        return;
+    }
+    shuffle_group_lds<SHUF_GW>(n, b, run, k0, k1, a, ent, hist, wsum);
+}
 
 // ---------------------------------------------------------------------------
 // Bit-exact device permutations: numpy's legacy RandomState(seed).permutation
@@ -2549,9 +2999,35 @@ extern "C" int pc_record_bytes(uint32_t num_nodes)
     return num_nodes <= kU32RecordMaxNodes ? 4 : 8;
 }
 
+// Bucket bits and chunks of the device permutations for num_edges edges.
+static void shuffle_plan(uint32_t num_edges, uint32_t &lb1, uint32_t &chunks)
+{
+    lb1 = 0;
+    while (((uint64_t)SHUF_BUCKET << lb1) < num_edges)
+        ++lb1;
+    chunks = (num_edges + SHUF_CHUNK - 1u) / SHUF_CHUNK;
+}
+
+extern "C" int pc_permute_philox_workspace_size(uint32_t num_edges,
+                                                uint32_t num_runs,
+                                                uint64_t *bytes)
+{
+    g_last_error[0] = 0;
+    if (!bytes)
+        return pc_fail(PC_EINVAL, "pc_permute_philox_workspace_size: bytes is NULL");
+    if (num_edges > SHUF_MAX_EDGES)
+        return pc_fail(PC_EOVERFLOW, "pc_permute_philox_workspace_size: num_edges > 2^23");
+    uint32_t lb1 = 0, chunks = 0;
+    shuffle_plan(num_edges, lb1, chunks);
+    const uint64_t nb1 = 1ull << lb1;
+    *bytes = num_edges ? (uint64_t)num_runs * (nb1 * chunks + nb1 + 1u) * 4u : 0u;
+    return PC_OK;
+}
+
 extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
                                  uint32_t num_runs, uint32_t num_edges,
                                  int32_t *d_perm, int64_t perm_stride,
+                                 void *d_workspace, uint64_t workspace_bytes,
                                  void *stream)
 {
     g_last_error[0] = 0;
@@ -2559,19 +3035,40 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
         return PC_OK;
     if (!d_perm || perm_stride < (int64_t)num_edges)
         return pc_fail(PC_EINVAL, "pc_permute_philox: bad output buffer");
-    if (num_edges > (1u << 24))
-        return pc_fail(PC_EOVERFLOW, "pc_permute_philox: num_edges > 2^24");
-    uint32_t bits = 1;
-    while ((1ull << bits) < num_edges)
-        ++bits;
-    const size_t lds = (size_t)(PERM_ROUNDS << ((bits + 1u) / 2u)) * sizeof(uint16_t);
-    const dim3 grid((num_edges + PERM_BLOCK * PERM_ITEMS - 1) /
-                        (PERM_BLOCK * PERM_ITEMS),
-                    num_runs);
-    permute_kernel<<<grid, PERM_BLOCK, lds, as_stream(stream)>>>(
-        (uint32_t)seed, (uint32_t)(seed >> 32), first_run, num_edges, bits,
-        d_perm, perm_stride);
-    PC_LAUNCH_CHECK("permute_kernel");
+    if (num_edges > SHUF_MAX_EDGES)
+        return pc_fail(PC_EOVERFLOW, "pc_permute_philox: num_edges > 2^23");
+    if (first_run + num_runs > (1ull << 32))
+        return pc_fail(PC_EOVERFLOW, "pc_permute_philox: run indices reach 2^32");
+    uint64_t need = 0;
+    pc_permute_philox_workspace_size(num_edges, num_runs, &need);
+    if (!d_workspace || workspace_bytes < need)
+        return pc_fail(PC_ENOMEM, "pc_permute_philox: workspace needs %llu bytes",
+                       (unsigned long long)need);
+    uint32_t lb1 = 0, chunks = 0;
+    shuffle_plan(num_edges, lb1, chunks);
+    const uint32_t nb1 = 1u << lb1;
+    uint32_t *counts = (uint32_t *)d_workspace;
+    uint32_t *starts = counts + (uint64_t)num_runs * nb1 * chunks;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t fr = (uint32_t)first_run;
+    hipStream_t s = as_stream(stream);
+    // runs in slabs of at most 65535 (the grid's y extent)
+    for (uint32_t r0 = 0; r0 < num_runs; r0 += 65535u) {
+        const uint32_t rn = min(num_runs - r0, 65535u);
+        shuffle_count_kernel<<<dim3(chunks, rn), 64 * SHUF_W, nb1 * 4u, s>>>(
+            k0, k1, fr, num_edges, lb1, chunks, r0, counts);
+        PC_LAUNCH_CHECK("shuffle_count_kernel");
+        shuffle_scan_kernel<<<rn, 256, nb1 * 4u, s>>>(num_edges, lb1, chunks, r0,
+                                                      counts, starts);
+        PC_LAUNCH_CHECK("shuffle_scan_kernel");
+        shuffle_scatter_kernel<<<dim3(chunks, rn), 64 * SHUF_W,
+                                 (SHUF_W + 2u) * nb1 * 4u, s>>>(
+            k0, k1, fr, num_edges, lb1, chunks, r0, counts, d_perm, perm_stride);
+        PC_LAUNCH_CHECK("shuffle_scatter_kernel");
+        shuffle_group_kernel<<<dim3(nb1, rn), 64 * SHUF_GW, 0, s>>>(
+            k0, k1, fr, lb1, r0, starts, d_perm, perm_stride);
+        PC_LAUNCH_CHECK("shuffle_group_kernel");
+    }
     return PC_OK;
 }
 

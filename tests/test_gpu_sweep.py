@@ -127,16 +127,22 @@ def test_global_state(gpu, length):
     _check_sums(eng, res, expect)
 
 
-@pytest.mark.parametrize('M', [1, 12, 480, 5000, 32512, 65536, 70000])
+@pytest.mark.parametrize('M', [1, 2, 12, 33, 480, 4096, 4097, 32512, 32769,
+                               65536, 65537, 130560, 2095104])
 def test_device_permutation_matches_oracle(gpu, M):
-    """Odd and even bit counts (1, 4, 9, 13, 15: the L = 128 lattice, 16 and
-    17 bits), against the oracle's restatement."""
+    """pc_permute_philox equals the oracle's restatement of the bucket
+    shuffle bit for bit: one bucket (M <= 4096), several (the L = 128
+    lattice: 8), 16- and 32-bit LDS entries (either side of 65536 edges),
+    several chunks of the count and scatter passes (the L = 256 and L = 1024
+    lattices)."""
     eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
         [np.arange(M), np.arange(1, M + 1)], axis=1)))
-    d = eng.device_permutations(seed=1234, num_runs=3, first_run=5)
+    runs = 3 if M < 1000000 else 2
+    d = eng.device_permutations(seed=0x1234567890AB, num_runs=runs, first_run=5)
     got = d.cpu().numpy()
-    for r in range(3):
-        np.testing.assert_array_equal(got[r], oracle.feistel_permutation(1234, 5 + r, M))
+    want = oracle.bucket_permutations(0x1234567890AB, 5, runs, M)
+    for r in range(runs):
+        np.testing.assert_array_equal(got[r], want[r], err_msg='run {}'.format(r))
 
 
 def test_device_permutation_L128_is_permutation(gpu):
