@@ -8,6 +8,7 @@ Each function restates the reference (andsor/pypercolate 0.4.6) step for
 step, citing the lines it follows:
 
 * ``percolation_graph_arrays``  percolate/percolate.py:31-100 (networkx 3 API)
+* ``square_lattice_graph``      the same for spanning_2d_grid(L), closed form
 * ``bond_runs``                 percolate/hpc.py:194-307 via nz_oracle.c
 * ``microcanonical_sums``       exact sums over runs (nz_oracle.c)
 * ``bond_sums``                 sweep + exact sums without traces (nz_oracle.c)
@@ -135,6 +136,33 @@ def percolation_graph_arrays(graph, spanning_cluster=True):
     ret['aux_edges'] = np.array(aux_edges, dtype=np.int64).reshape(-1, 2)
     ret['aux_edge_side'] = np.array(aux_edge_side, dtype=np.int32)
     return ret
+
+
+def square_lattice_graph(length):
+    """percolation_graph_arrays(spanning_2d_grid(length)) without networkx
+    (which takes minutes and GBs at L = 1024): perc node (i, j), i = 1..L,
+    j = 0..L-1, gets index (i-1) L + j; the edges in networkx's order for
+    the grid's perc subgraph (row by row, each node's edge down, then
+    right); one auxiliary node per column on each side, joined to the first
+    and last row (percolate/percolate.py:931-965).  tests/test_oracle_*
+    pin it to the networkx-built arrays on small lattices."""
+    L = int(length)
+    N = L * L
+    edges = []
+    for i in range(L):
+        for j in range(L):
+            u = i * L + j
+            if i + 1 < L:
+                edges.append((u, u + L))
+            if j + 1 < L:
+                edges.append((u, u + 1))
+    return {'num_nodes': N, 'num_edges': len(edges), 'spanning': True,
+            'edges': np.array(edges, dtype=np.int64).reshape(-1, 2),
+            'aux_side': np.array([0] * L + [1] * L, dtype=np.int32),
+            'aux_edges': np.array([(j, N + j) for j in range(L)] +
+                                  [((L - 1) * L + j, N + L + j) for j in range(L)],
+                                  dtype=np.int64).reshape(-1, 2),
+            'aux_edge_side': np.array([0] * L + [1] * L, dtype=np.int32)}
 
 
 def _edge_span_attributes(graph):

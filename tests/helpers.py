@@ -55,18 +55,8 @@ def ratio_weights(n, p, k):
 
 
 def lattice_oracle_graph(length):
-    """The oracle's graph dict of spanning_2d_grid(length), built from the
-    closed-form lattice (whose edge order tests/test_graph_cpu.py pins to
-    networkx's) instead of a networkx graph, which takes minutes and GBs at
-    L = 1024.  Auxiliary nodes: one per column on each side, attached to the
-    first and last row, as percolate/percolate.py:931-965 builds them."""
-    from pypercolate_amd import _graph
-    ga = _graph.square_lattice(length)
-    L, N = int(length), ga.num_nodes
-    return {'num_nodes': N, 'num_edges': ga.num_edges, 'spanning': True,
-            'edges': ga.edges.astype(np.int64),
-            'aux_side': np.array([0] * L + [1] * L, dtype=np.int32),
-            'aux_edges': np.array([(i, N + i) for i in range(L)] +
-                                  [(N - L + i, N + L + i) for i in range(L)],
-                                  dtype=np.int64).reshape(-1, 2),
-            'aux_edge_side': np.array([0] * L + [1] * L, dtype=np.int32)}
+    """The oracle's graph dict of spanning_2d_grid(length) in closed form
+    (oracle.square_lattice_graph, pinned to networkx on small lattices by
+    tests/test_oracle_exact.py)."""
+    from oracle import oracle
+    return oracle.square_lattice_graph(length)

@@ -181,3 +181,17 @@ def test_microcanonical_step_stats_catch_a_planted_error():
     wrong = scipy.stats.t.interval(1 - alpha, df=runs, loc=m, scale=sd / np.sqrt(runs))
     cdf = scipy.stats.t.cdf(wrong, df=runs - 1, loc=m, scale=sd / np.sqrt(runs))
     assert np.abs(cdf - [alpha / 2, 1 - alpha / 2]).max() > 1e-4
+
+
+@pytest.mark.parametrize('length', [1, 2, 3, 7, 12])
+def test_square_lattice_graph_equals_networkx(length):
+    """The oracle's closed-form lattice gives the same traces as the
+    networkx-built graph (same edge order, same sides)."""
+    a = oracle.square_lattice_graph(length)
+    b = oracle.percolation_graph_arrays(spanning_2d_grid(length), True)
+    assert a['num_nodes'] == b['num_nodes'] and a['num_edges'] == b['num_edges']
+    np.testing.assert_array_equal(a['edges'], b['edges'])
+    perms = oracle.randomstate_permutations(range(6), a['num_edges'])
+    ta, tb = oracle.bond_runs(a, perms), oracle.bond_runs(b, perms)
+    for k in ta:
+        np.testing.assert_array_equal(ta[k], tb[k])
