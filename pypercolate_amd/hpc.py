@@ -266,27 +266,43 @@ def bond_canonical_statistics(
     return ret
 
 
+# The averaged statistics: (field of the canonical statistics, name in the
+# finalized averages, divided by the number of nodes when finalized).
+_STATISTICS = (('percolation_probability', 'percolation_probability', False),
+               ('max_cluster_size', 'percolation_strength', True),
+               ('moments', 'moments', True))
+
+
+def _statistics_in(names, suffixes=('',)):
+    """The entries of _STATISTICS whose fields (with every suffix) are all
+    among the dtype field names."""
+    return [st for st in _STATISTICS
+            if all(st[0] + sfx in names for sfx in suffixes)]
+
+
+def _per_row(values, like):
+    """A per-row (per-p) array as float64, shaped to broadcast against a
+    field with more axes (the moments' 5)."""
+    v = np.asarray(values, dtype=np.float64)
+    return v.reshape(v.shape + (1,) * (np.ndim(like) - v.ndim))
+
+
 def bond_initialize_canonical_averages(
     canonical_statistics, **kwargs
 ):
     """Canonical averages of a single run (hpc.py:561-635): number_of_runs 1,
-    means equal to the statistics, m2 zero."""
-    spanning_cluster = (
-        'percolation_probability' in canonical_statistics.dtype.names
-    )
+    every statistic its own mean, with zero sum of squared deviations.
+    Spanning is detected from the field names, as in the reference."""
+    names = canonical_statistics.dtype.names
     ret = np.empty_like(
         canonical_statistics,
-        dtype=canonical_averages_dtype(spanning_cluster=spanning_cluster),
+        dtype=canonical_averages_dtype(
+            spanning_cluster='percolation_probability' in names),
     )
     ret['number_of_runs'] = 1
-    if spanning_cluster:
-        ret['percolation_probability_mean'] = (
-            canonical_statistics['percolation_probability'])
-        ret['percolation_probability_m2'] = 0.0
-    ret['max_cluster_size_mean'] = canonical_statistics['max_cluster_size']
-    ret['max_cluster_size_m2'] = 0.0
-    ret['moments_mean'] = canonical_statistics['moments']
-    ret['moments_m2'] = 0.0
+    for field, _, _ in _statistics_in(names):
+        ret[field + '_mean'] = canonical_statistics[field]
+        ret[field + '_m2'] = 0.0
     return ret
 
 
@@ -306,31 +322,16 @@ def _online_variance(a, b):
 
 def bond_reduce(row_a, row_b):
     """Reduce the canonical averages over several runs (hpc.py:638-702):
-    associative and commutative merge of (n, mean, M2) per field."""
-    spanning_cluster = (
-        'percolation_probability_mean' in row_a.dtype.names and
-        'percolation_probability_mean' in row_b.dtype.names and
-        'percolation_probability_m2' in row_a.dtype.names and
-        'percolation_probability_m2' in row_b.dtype.names
-    )
+    associative and commutative merge of (n, mean, M2) for every statistic
+    whose mean and M2 both rows carry."""
+    both = [st for st in _statistics_in(row_a.dtype.names, ('_mean', '_m2'))
+            if st in _statistics_in(row_b.dtype.names, ('_mean', '_m2'))]
     ret = np.empty_like(row_a)
-
-    def _reducer(key, transpose=False):
-        mean_key = '{}_mean'.format(key)
-        m2_key = '{}_m2'.format(key)
-        na = row_a['number_of_runs']
-        nb = row_b['number_of_runs']
-        if transpose:
-            na = na[..., np.newaxis]
-            nb = nb[..., np.newaxis]
-        res = _online_variance((na, row_a[mean_key], row_a[m2_key]),
-                               (nb, row_b[mean_key], row_b[m2_key]))
-        ret[mean_key], ret[m2_key] = res[1], res[2]
-
-    if spanning_cluster:
-        _reducer('percolation_probability')
-    _reducer('max_cluster_size')
-    _reducer('moments', transpose=True)
+    for field, _, _ in both:
+        mean, m2 = field + '_mean', field + '_m2'
+        _, ret[mean], ret[m2] = _online_variance(
+            (_per_row(row_a['number_of_runs'], row_a[mean]), row_a[mean], row_a[m2]),
+            (_per_row(row_b['number_of_runs'], row_b[mean]), row_b[mean], row_b[m2]))
     ret['number_of_runs'] = row_a['number_of_runs'] + row_b['number_of_runs']
     return ret
 
@@ -338,51 +339,34 @@ def bond_reduce(row_a, row_b):
 def finalize_canonical_averages(
     number_of_nodes, ps, canonical_averages, alpha,
 ):
-    """Finalize canonical averages (hpc.py:752-834): mean, sample standard
-    deviation and t confidence interval; max_cluster_size becomes the
-    percolation strength and the moments are divided by the number of
-    nodes."""
-    spanning_cluster = (
-        ('percolation_probability_mean' in canonical_averages.dtype.names) and
-        'percolation_probability_m2' in canonical_averages.dtype.names
-    )
+    """Finalize canonical averages (hpc.py:752-834): per statistic the
+    mean, the sample standard deviation sqrt(M2 / (n - 1)) and the Student-t
+    confidence interval of the mean; max_cluster_size becomes the
+    percolation strength, and it and the moments are divided by the number
+    of nodes."""
+    names = canonical_averages.dtype.names
+    present = _statistics_in(names, ('_mean', '_m2'))
+    spanning = any(st[0] == 'percolation_probability' for st in present)
     ret = np.empty_like(
         canonical_averages,
-        dtype=finalized_canonical_averages_dtype(
-            spanning_cluster=spanning_cluster),
+        dtype=finalized_canonical_averages_dtype(spanning_cluster=spanning),
     )
-    n = canonical_averages['number_of_runs']
-    sqrt_n = np.sqrt(canonical_averages['number_of_runs'])
-    ret['number_of_runs'] = n
+    runs = canonical_averages['number_of_runs']
+    ret['number_of_runs'] = runs
     ret['p'] = ps
     ret['alpha'] = alpha
-
-    def _transform(original_key, final_key=None, normalize=False,
-                   transpose=False):
-        if final_key is None:
-            final_key = original_key
-        mean = canonical_averages['{}_mean'.format(original_key)]
-        m2 = canonical_averages['{}_m2'.format(original_key)]
-        nn = n[..., np.newaxis] if transpose else n
-        sq = sqrt_n[..., np.newaxis] if transpose else sqrt_n
-        mean_out = np.array(mean, dtype=np.float64)
-        if normalize:
-            mean_out = mean_out / number_of_nodes
-        std = np.sqrt(m2 / (nn - 1))
-        if normalize:
+    for field, final, per_node in present:
+        mean = np.array(canonical_averages[field + '_mean'], dtype=np.float64)
+        n = _per_row(runs, mean)
+        std = np.sqrt(canonical_averages[field + '_m2'] / (n - 1))
+        if per_node:
+            mean = mean / number_of_nodes
             std = std / number_of_nodes
-        ret['{}_mean'.format(final_key)] = mean_out
-        ret['{}_std'.format(final_key)] = std
-        lo, hi = scipy.stats.t.interval(1 - alpha, df=nn - 1, loc=mean_out,
-                                        scale=std / sq)
-        ci = ret['{}_ci'.format(final_key)]
-        ci[..., 0] = lo
-        ci[..., 1] = hi
-
-    if spanning_cluster:
-        _transform('percolation_probability')
-    _transform('max_cluster_size', 'percolation_strength', normalize=True)
-    _transform('moments', normalize=True, transpose=True)
+        lower, upper = scipy.stats.t.interval(
+            1 - alpha, df=n - 1, loc=mean, scale=std / np.sqrt(n))
+        ret[final + '_mean'] = mean
+        ret[final + '_std'] = std
+        ret[final + '_ci'] = np.stack([lower, upper], axis=-1)
     return ret
 
 

@@ -8,10 +8,11 @@ arithmetic are listed where they occur:
 
 * graphs use networkx 3 (``graph.nodes[n]`` for ``graph.node[n]``), and edge
   order is ``perc_graph.edges()`` of the installed networkx;
-* moments are exact integers converted to float64 at the end; the reference
-  accumulates them in float64 (percolate.py:259,334-350), which agrees
-  exactly while values stay below 2**53 (every lattice up to L = 128) and
-  overflows int64 for clusters above 55108 nodes.
+* moments are exact integers (128-bit) rounded to float64 at the end; the
+  reference accumulates them in float64 with int64 powers
+  (percolate.py:259,334-350), which agrees exactly while values stay below
+  2**53 (every lattice up to L = 128) and wraps int64 for clusters above
+  55108 nodes.
 """
 
 import copy
@@ -28,38 +29,46 @@ The alpha for the 1 sigma confidence level
 """
 
 
+_NO_AUX = ('Spanning cluster is to be detected, but no auxiliary nodes '
+           'given.')
+_NOT_TWO_SIDES = ('Spanning cluster is to be detected, but auxiliary nodes '
+                  'of less or more than 2 types (sides) given.')
+
+
+def _auxiliary_sides(graph):
+    """The auxiliary nodes of a spanning-cluster graph (those with a 'span'
+    attribute) as {node: side}, and the list of distinct sides; ValueError
+    (the reference's messages) unless there are nodes of exactly two
+    sides."""
+    node_side = {node: data['span'] for node, data in graph.nodes(data=True)
+                 if 'span' in data}
+    if not node_side:
+        raise ValueError(_NO_AUX)
+    sides = list(set(node_side.values()))
+    if len(sides) != 2:
+        raise ValueError(_NOT_TWO_SIDES)
+    return node_side, sides
+
+
 def percolation_graph(graph, spanning_cluster=True):
-    """Prepare the (internal) percolation graph from a given graph
-    (percolate/percolate.py:31-100): strips the auxiliary nodes and returns
-    the keyword dict the ``percolate.hpc`` functions take."""
-    ret = dict()
-    ret['graph'] = graph
-    ret['spanning_cluster'] = bool(spanning_cluster)
-    if spanning_cluster:
-        spanning_auxiliary_node_attributes = nx.get_node_attributes(graph, 'span')
-        ret['auxiliary_node_attributes'] = spanning_auxiliary_node_attributes
-        if not list(spanning_auxiliary_node_attributes.keys()):
-            raise ValueError(
-                'Spanning cluster is to be detected, but no auxiliary nodes '
-                'given.'
-            )
-        spanning_sides = list(set(spanning_auxiliary_node_attributes.values()))
-        if len(spanning_sides) != 2:
-            raise ValueError(
-                'Spanning cluster is to be detected, but auxiliary nodes '
-                'of less or more than 2 types (sides) given.'
-            )
-        ret['spanning_sides'] = spanning_sides
-        ret['auxiliary_edge_attributes'] = nx.get_edge_attributes(graph, 'span')
-        perc_graph = graph.subgraph(
-            [node for node in graph.nodes() if 'span' not in graph.nodes[node]]
-        )
-    else:
-        perc_graph = graph
-    ret['perc_graph'] = perc_graph
-    ret['num_nodes'] = nx.number_of_nodes(perc_graph)
-    ret['num_edges'] = nx.number_of_edges(perc_graph)
-    return ret
+    """The percolation graph of a graph and the keyword dict the
+    ``percolate.hpc`` functions take (percolate/percolate.py:31-100): with
+    spanning detection the auxiliary nodes (attribute 'span') are left out
+    of ``perc_graph`` and described by ``auxiliary_node_attributes``,
+    ``auxiliary_edge_attributes`` and the two ``spanning_sides``."""
+    detect = bool(spanning_cluster)
+    keywords = {'graph': graph, 'spanning_cluster': detect}
+    perc_graph = graph
+    if detect:
+        node_side, sides = _auxiliary_sides(graph)
+        keywords['auxiliary_node_attributes'] = node_side
+        keywords['spanning_sides'] = sides
+        keywords['auxiliary_edge_attributes'] = nx.get_edge_attributes(graph, 'span')
+        perc_graph = graph.subgraph([n for n in graph.nodes() if n not in node_side])
+    keywords['perc_graph'] = perc_graph
+    keywords['num_nodes'] = perc_graph.number_of_nodes()
+    keywords['num_edges'] = perc_graph.number_of_edges()
+    return keywords
 
 
 def spanning_1d_chain(length):
@@ -95,17 +104,7 @@ def _check_model_and_sides(graph, spanning_cluster, model):
     if model != 'bond':
         raise ValueError('Only bond percolation supported.')
     if spanning_cluster:
-        aux = nx.get_node_attributes(graph, 'span')
-        if not list(aux.keys()):
-            raise ValueError(
-                'Spanning cluster is to be detected, but no auxiliary nodes '
-                'given.'
-            )
-        if len(set(aux.values())) != 2:
-            raise ValueError(
-                'Spanning cluster is to be detected, but auxiliary nodes '
-                'of less or more than 2 types (sides) given.'
-            )
+        _auxiliary_sides(graph)
 
 
 def _engine(graph_arrays):
@@ -119,6 +118,31 @@ def _run_batch(ga, perms):
     eng = _engine(ga)
     res = eng.sweep(eng.upload_permutations(perms))
     return eng, res
+
+
+# largest graph whose exact per-step sums pc_microcanonical_sums takes
+_SUMS_MAX_NODES = 1 << 21
+
+
+def _single_run(ga, perm):
+    """One run with the host permutation perm: largest cluster (int64),
+    spanning flag (bool) and the five moments as float64, (S,) and (S, 5).
+    The moments are the run's exact integers rounded to float64 (within one
+    unit in the last place; the reference sums them in float64 with int64
+    powers, percolate/percolate.py:259,334-350, which agrees while they stay
+    below 2**53 and wraps for clusters above 55108 nodes): for one run the
+    exact 128-bit per-step sums are the run's own values.  Graphs above
+    2**21 nodes take the per-run traces, whose moments are kept mod 2**64."""
+    eng, res = _run_batch(ga, perm[np.newaxis, :])
+    if ga.num_nodes <= _SUMS_MAX_NODES:
+        s1, _ = eng.microcanonical_sums(res)
+        limbs = s1.cpu().numpy().view(np.uint64)          # (7, S, 2)
+        exact = limbs[:, :, 0].astype(np.float64) + \
+            limbs[:, :, 1].astype(np.float64) * 18446744073709551616.0
+        return (limbs[0, :, 0].astype(np.int64), limbs[1, :, 0] != 0,
+                exact[2:].T.copy())
+    mx, sp, mom = eng.host_traces(res)
+    return mx[0].astype(np.int64), sp[0], mom[0].astype(np.float64)
 
 
 def sample_states(
@@ -154,19 +178,17 @@ def sample_states(
     perm_edges = np.random.permutation(M)
     if M == 0:
         return
-    eng, res = _run_batch(ga, perm_edges[np.newaxis, :])
-    mx, sp, mom = eng.host_traces(res)
-    moments = mom[0].astype(np.float64)
+    mx, sp, moments = _single_run(ga, perm_edges)
     for n in range(1, M + 1):
         ret['n'] = n
         ret['edge'] = ga.edge_labels(perm_edges[n - 1])
-        ret['max_cluster_size'] = int(mx[0, n])
+        ret['max_cluster_size'] = int(mx[n])
         if copy_result:
             ret['moments'] = moments[n].copy()
         else:
             ret['moments'][:] = moments[n]
         if spanning_cluster:
-            ret['has_spanning_cluster'] = bool(sp[0, n])
+            ret['has_spanning_cluster'] = bool(sp[n])
         if copy_result:
             yield copy.deepcopy(ret)
         else:
@@ -189,16 +211,15 @@ def single_run_arrays(spanning_cluster=True, **kwargs):
     perm = np.random.permutation(M)
     ret = {'N': N, 'M': M}
     if M:
-        eng, res = _run_batch(ga, perm[np.newaxis, :])
-        mx, sp, mom = eng.host_traces(res)
+        mx, sp, mom = _single_run(ga, perm)
     else:
-        mx = np.ones((1, 1), dtype=np.uint32)
-        sp = np.zeros((1, 1), dtype=bool)
-        mom = np.full((1, 1, 5), N - 1, dtype=np.uint64)
-    ret['max_cluster_size'] = mx[0].astype(np.float64)
-    ret['moments'] = mom[0].T.astype(np.float64)
+        mx = np.ones(1, dtype=np.int64)
+        sp = np.zeros(1, dtype=bool)
+        mom = np.full((1, 5), N - 1.0)
+    ret['max_cluster_size'] = mx.astype(np.float64)
+    ret['moments'] = mom.T.copy()
     if spanning_cluster:
-        ret['has_spanning_cluster'] = sp[0].astype(bool)
+        ret['has_spanning_cluster'] = sp.astype(bool)
     return ret
 
 
@@ -244,10 +265,12 @@ class _MicrocanonicalStats(object):
         self.runs, self.alpha = runs, alpha
         self.spanning = spanning_cluster
         S = self.M + 1
+        self.device_out = None
         if self.M:
             eng, res = _run_batch(ga, perms)
             s1, s2 = eng.microcanonical_sums(res)
-            out = E.micro_ci(s1, s2, self.M, runs, alpha).cpu().numpy()
+            self.device_out = E.micro_ci(s1, s2, self.M, runs, alpha)
+            out = self.device_out.cpu().numpy()
             mom = [6, 9, 12, 15, 18]
             self.max_mean = out[0]
             self.max_ci = np.stack([out[1], out[2]], axis=-1)
@@ -281,6 +304,36 @@ class _MicrocanonicalStats(object):
             ret['spanning_cluster'] = self.span_mean[n]
             ret['spanning_cluster_ci'] = self.span_ci[n].copy()
         return ret
+
+    # rows of pc_microcanonical_ci's output in the column order of
+    # canonical_averages over arrays(): max, its bounds, moments 0..4, their
+    # (lower, upper) bounds, then spanning and its bounds
+    _DEVICE_COLUMNS = [0, 1, 2, 6, 9, 12, 15, 18, 7, 8, 10, 11, 13, 14, 16, 17,
+                       19, 20]
+    _DEVICE_LAYOUT = [('max_cluster_size', 0, 1), ('max_cluster_size_ci', 1, 2),
+                      ('moments', 3, 5), ('moments_ci', 8, 10)]
+
+    def device_columns(self):
+        """The (S, C) device matrix canonical_averages builds from arrays(),
+        made on the device (the same values: the divisions by N are the same
+        IEEE operations), and its column layout; None without edges."""
+        if self.device_out is None:
+            return None
+        rows = list(self._DEVICE_COLUMNS)
+        layout = list(self._DEVICE_LAYOUT)
+        picked = self.device_out[rows]
+        # element-wise true division by a tensor of N (torch turns division
+        # by a Python scalar into a multiplication by its reciprocal, which
+        # is not the host's numpy division)
+        scaled = picked / picked.new_full(picked.shape, float(self.N))
+        if self.spanning:
+            x = self.device_out.new_empty((len(rows) + 3, picked.shape[1]))
+            x[:len(rows)] = scaled
+            x[len(rows):] = self.device_out[[3, 4, 5]]
+            layout += [('spanning_cluster', 18, 1), ('spanning_cluster_ci', 19, 2)]
+        else:
+            x = scaled
+        return x.t().contiguous(), layout
 
     def arrays(self):
         """microcanonical_averages_arrays of all steps (percolate.py:968-1064)."""
@@ -381,6 +434,18 @@ class _MicrocanonicalAverages(object):
 
     next = __next__
 
+    def _all_stats(self):
+        """All steps' statistics of a fresh iterator (or None)."""
+        if self._n is not None:
+            return None
+        self._start()
+        self._n = 0
+        if self._ga.num_edges == 0:
+            return None
+        self._compute()
+        self._n = self._ga.num_edges
+        return self._stats
+
     def _all_arrays(self):
         """Fast path for microcanonical_averages_arrays on a fresh iterator."""
         if self._n is not None:
@@ -407,42 +472,39 @@ def microcanonical_averages(
                                    alpha, copy_result)
 
 
+# the per-step entries microcanonical_averages_arrays stacks, with the axis
+# the steps take in the result (the moments keep their 5 rows first)
+_STEP_ARRAYS = (('max_cluster_size', 0), ('max_cluster_size_ci', 0),
+                ('spanning_cluster', 0), ('spanning_cluster_ci', 0),
+                ('moments', 1), ('moments_ci', 1))
+
+
 def microcanonical_averages_arrays(microcanonical_averages):
     """Compile microcanonical averages over all steps into arrays, all but
-    the spanning statistics divided by N (percolate/percolate.py:968-1064)."""
+    the spanning statistics divided by N (percolate/percolate.py:968-1064).
+
+    A fresh iterator of this package's microcanonical_averages is compiled
+    on the device without visiting the steps; any other iterator of
+    per-step dicts is stacked entry by entry."""
     fast = getattr(microcanonical_averages, '_all_arrays', None)
     if fast is not None:
         arrays = fast()
         if arrays is not None:
             return arrays
-    ret = dict()
-    for n, microcanonical_average in enumerate(microcanonical_averages):
-        assert n == microcanonical_average['n']
-        if n == 0:
-            num_edges = microcanonical_average['M']
-            num_sites = microcanonical_average['N']
-            spanning_cluster = ('spanning_cluster' in microcanonical_average)
-            ret['max_cluster_size'] = np.empty(num_edges + 1)
-            ret['max_cluster_size_ci'] = np.empty((num_edges + 1, 2))
-            if spanning_cluster:
-                ret['spanning_cluster'] = np.empty(num_edges + 1)
-                ret['spanning_cluster_ci'] = np.empty((num_edges + 1, 2))
-            ret['moments'] = np.empty((5, num_edges + 1))
-            ret['moments_ci'] = np.empty((5, num_edges + 1, 2))
-        ret['max_cluster_size'][n] = microcanonical_average['max_cluster_size']
-        ret['max_cluster_size_ci'][n] = microcanonical_average['max_cluster_size_ci']
-        if spanning_cluster:
-            ret['spanning_cluster'][n] = microcanonical_average['spanning_cluster']
-            ret['spanning_cluster_ci'][n] = (
-                microcanonical_average['spanning_cluster_ci'])
-        ret['moments'][:, n] = microcanonical_average['moments']
-        ret['moments_ci'][:, n] = microcanonical_average['moments_ci']
-    for key in ret:
-        if 'spanning_cluster' in key:
+    steps = list(microcanonical_averages)
+    if [st['n'] for st in steps] != list(range(len(steps))):
+        raise AssertionError('microcanonical averages out of step order')
+    num_nodes, num_edges = steps[0]['N'], steps[0]['M']
+    ret = {}
+    for key, axis in _STEP_ARRAYS:
+        if key not in steps[0]:
             continue
-        ret[key] /= num_sites
+        stacked = np.moveaxis(
+            np.stack([np.asarray(st[key], dtype=np.float64) for st in steps]),
+            0, axis)
+        ret[key] = stacked if key.startswith('spanning') else stacked / num_nodes
     ret['M'] = num_edges
-    ret['N'] = num_sites
+    ret['N'] = num_nodes
     return ret
 
 
@@ -465,27 +527,10 @@ _CANONICAL_KEYS = ('max_cluster_size', 'max_cluster_size_ci',
                    'moments', 'moments_ci')
 
 
-def canonical_averages(ps, microcanonical_averages_arrays):
-    """Canonical cluster statistics from microcanonical ones
-    (percolate/percolate.py:1112-1223): every array (means and confidence
-    bounds) convolved with the binomial weights of each p, on the GPU in
-    FP64 with the weights generated inside the contraction
-    (pc_canonical_fused)."""
-    from . import _engine as E
-    import torch
-    arrays = microcanonical_averages_arrays
-    num_sites = arrays['N']
-    num_edges = arrays['M']
-    spanning_cluster = ('spanning_cluster' in arrays)
-    ps = np.asarray(ps)
-
-    ret = dict()
-    ret['ps'] = ps
-    ret['N'] = num_sites
-    ret['M'] = num_edges
-
-    cols = []
-    layout = []
+def _canonical_columns(arrays):
+    """The (S, C) matrix of every microcanonical array, one column per
+    convolved series, and its layout [(key, first column, width)]."""
+    cols, layout, at = [], [], 0
     for key, value in arrays.items():
         if len(key) <= 1:
             continue
@@ -493,46 +538,78 @@ def canonical_averages(ps, microcanonical_averages_arrays):
             raise NotImplementedError('{}-dimensional array'.format(
                 np.ndim(value)))
         v = np.asarray(value, dtype=np.float64)
-        if key in ('max_cluster_size', 'spanning_cluster'):
-            block = v.reshape(-1, 1)
-        elif key in ('max_cluster_size_ci', 'spanning_cluster_ci'):
-            block = v.reshape(-1, 2)
-        elif key == 'moments':
-            block = v.T
-        else:  # moments_ci (5, S, 2) -> (S, 10)
-            block = v.transpose(1, 0, 2).reshape(v.shape[1], -1)
-        layout.append((key, len(cols) and sum(c.shape[1] for c in cols), block.shape[1]))
+        if key.endswith('_ci'):
+            # (S, 2), or moments (5, S, 2) -> (S, 10) as (lower, upper) pairs
+            block = v.reshape(-1, 2) if v.ndim == 2 else \
+                v.transpose(1, 0, 2).reshape(v.shape[1], -1)
+        else:
+            block = v.reshape(-1, 1) if v.ndim == 1 else v.T
+        layout.append((key, at, block.shape[1]))
+        at += block.shape[1]
         cols.append(block)
-    x = np.ascontiguousarray(np.concatenate(cols, axis=1)) if cols else \
-        np.zeros((num_edges + 1, 0))
-    if ps.size and x.shape[1]:
-        dx = torch.from_numpy(x).to(E.nat.torch_cuda().device('cuda'))
-        q = E.canonical_fused(dx, ps.reshape(-1), 'fp64_mfma').cpu().numpy()
-    else:
-        q = np.zeros((ps.size, x.shape[1]))
+    return cols, layout
+
+
+def _canonical_unpack(q, layout, ps, num_sites, num_edges):
+    """canonical_averages' result dict from the (P, C) convolved columns."""
+    ret = {'ps': ps, 'N': num_sites, 'M': num_edges}
     for key, off, width in layout:
         block = q[:, off:off + width]
-        if key in ('max_cluster_size', 'spanning_cluster'):
-            ret[key] = block[:, 0].copy()
-        elif key in ('max_cluster_size_ci', 'spanning_cluster_ci'):
-            ret[key] = block.copy()
-        elif key == 'moments':
+        if key == 'moments':
             ret[key] = block.T.copy()
-        else:
+        elif key == 'moments_ci':
             ret[key] = block.reshape(ps.size, 5, 2).transpose(1, 0, 2).copy()
+        elif width == 1:
+            ret[key] = block[:, 0].copy()
+        else:
+            ret[key] = block.copy()
     return ret
+
+
+def _convolve(x, ps):
+    """(P, C) host array: the fused FP64 canonical convolution of the
+    columns of the (S, C) device matrix x at every p (pc_canonical_fused)."""
+    from . import _engine as E
+    if ps.size and x.shape[1]:
+        return E.canonical_fused(x, ps.reshape(-1), 'fp64_mfma').cpu().numpy()
+    return np.zeros((ps.size, x.shape[1]))
+
+
+def canonical_averages(ps, microcanonical_averages_arrays):
+    """Canonical cluster statistics from microcanonical ones
+    (percolate/percolate.py:1112-1223): every array (means and confidence
+    bounds) convolved with the binomial weights of each p, on the GPU in
+    FP64 with the weights generated inside the contraction
+    (pc_canonical_fused)."""
+    from . import _engine as E
+    arrays = microcanonical_averages_arrays
+    ps = np.asarray(ps)
+    cols, layout = _canonical_columns(arrays)
+    S = arrays['M'] + 1
+    x = np.ascontiguousarray(np.concatenate(cols, axis=1)) if cols else \
+        np.zeros((S, 0))
+    torch = E.nat.torch_cuda()
+    q = _convolve(torch.from_numpy(x).to(torch.device('cuda')), ps)
+    return _canonical_unpack(q, layout, ps, arrays['N'], arrays['M'])
 
 
 def statistics(
     graph, ps, spanning_cluster=True, model='bond', alpha=alpha_1sigma, runs=40
 ):
-    """Helper function to compute percolation statistics
-    (percolate/percolate.py:1226-1252)."""
-    my_microcanonical_averages = microcanonical_averages(
+    """Canonical cluster statistics of a graph at every p
+    (percolate/percolate.py:1226-1252): the microcanonical averages over
+    ``runs`` runs convolved with the binomial weights.  The microcanonical
+    statistics stay on the device between the two stages (the per-step
+    arrays are never built on the host); the results equal those of
+    microcanonical_averages -> microcanonical_averages_arrays ->
+    canonical_averages bit for bit."""
+    averages = microcanonical_averages(
         graph=graph, runs=runs, spanning_cluster=spanning_cluster, model=model,
-        alpha=alpha
-    )
-    my_microcanonical_averages_arrays = microcanonical_averages_arrays(
-        my_microcanonical_averages
-    )
-    return canonical_averages(ps, my_microcanonical_averages_arrays)
+        alpha=alpha)
+    ps = np.asarray(ps)
+    stats = averages._all_stats()
+    cols = None if stats is None else stats.device_columns()
+    if cols is None:
+        return canonical_averages(ps, microcanonical_averages_arrays(averages))
+    x, layout = cols
+    return _canonical_unpack(_convolve(x, ps), layout, ps, stats.N, stats.M)

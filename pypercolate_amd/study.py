@@ -11,15 +11,17 @@ jugfile.py:57-135), reduce the tasks with ``hpc.bond_reduce``
 dataset per L named ``"{L}"`` (jugfile.py:138-156).
 
 Each task's reduced averages are written to ``checkpoint_dir`` as it
-finishes (``L{L}_task{t}.npy``, plain .npy with no pickled objects); a
-study run again with the same checkpoint directory loads the finished
-tasks instead of recomputing them.
+finishes (``L{L}_task{t}_{key}.npy``, plain .npy with no pickled objects,
+``key`` a digest of the parameters the task depends on); a study run again
+with the same parameters and checkpoint directory loads the finished tasks
+instead of recomputing them.
 
 Output: HDF5 (``percolate_hpc.hdf5``, dataset ``"{L}"``, as the reference
 writes it) when h5py is importable, otherwise an .npz archive with one
 array per L under the same name.
 """
 
+import hashlib
 import os
 from functools import reduce
 
@@ -41,8 +43,20 @@ def study_seeds(dimensions, runs, seed=STUDY_SEED):
     return {L: rng.randint(UINT32_MAX, size=runs) for L in dimensions}
 
 
-def _task_path(checkpoint_dir, L, t):
-    return os.path.join(checkpoint_dir, 'L{}_task{}.npy'.format(L, t))
+def task_key(ps, seeds, tasks, spanning_cluster):
+    """Short digest of everything a task's result depends on (the p grid,
+    the size's run seeds, the task split and spanning detection), part of
+    every checkpoint's name: a study rerun with other parameters in the same
+    directory computes its own tasks instead of loading stale ones."""
+    h = hashlib.sha1()
+    h.update(np.ascontiguousarray(ps, dtype=np.float64).tobytes())
+    h.update(np.ascontiguousarray(seeds, dtype=np.int64).tobytes())
+    h.update('{}:{}'.format(int(tasks), bool(spanning_cluster)).encode())
+    return h.hexdigest()[:16]
+
+
+def _task_path(checkpoint_dir, L, t, key):
+    return os.path.join(checkpoint_dir, 'L{}_task{}_{}.npy'.format(L, t, key))
 
 
 def run_study(dimensions=(8, 16, 32), runs=10000, tasks=100,
@@ -68,10 +82,17 @@ def run_study(dimensions=(8, 16, 32), runs=10000, tasks=100,
         # the graph's index arrays, read once per size rather than per task
         ga = _graph.from_percolation_graph(**pg)
         reduced = []
+        key = task_key(ps, seeds[L], tasks, spanning_cluster)
+        want = np.dtype(hpc.canonical_averages_dtype(spanning_cluster))
         for t, chunk in enumerate(np.split(seeds[L], tasks)):
-            path = _task_path(checkpoint_dir, L, t) if checkpoint_dir else None
+            path = _task_path(checkpoint_dir, L, t, key) if checkpoint_dir else None
             if path and os.path.exists(path):
-                reduced.append(np.load(path, allow_pickle=False))
+                done = np.load(path, allow_pickle=False)
+                if done.dtype != want or done.shape != ps.shape or \
+                        (done['number_of_runs'] != len(chunk)).any():
+                    raise ValueError('checkpoint {} does not belong to this '
+                                     'study'.format(path))
+                reduced.append(done)
                 continue
             avg = hpc.bond_canonical_averages_batch(ps=ps, seeds=chunk,
                                                     graph_arrays=ga, **pg)

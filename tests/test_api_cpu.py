@@ -129,3 +129,33 @@ def test_batch_graph_arrays_must_match_spanning():
     with pytest.raises(ValueError):
         hpc.bond_canonical_averages_batch(
             ps=[0.5], num_runs=1, graph_arrays=_graph.square_lattice(5), **pg)
+
+
+def test_arrays_from_a_plain_iterator_of_steps():
+    """microcanonical_averages_arrays over any iterator of per-step dicts
+    (percolate/percolate.py:968-1064): each entry stacked over the steps,
+    moments with their 5 rows first, all but spanning divided by N."""
+    from pypercolate_amd import percolate
+    rng = np.random.RandomState(0)
+    N, M = 9, 12
+    steps = []
+    for n in range(M + 1):
+        steps.append({'n': n, 'N': N, 'M': M,
+                      'max_cluster_size': rng.rand(),
+                      'max_cluster_size_ci': rng.rand(2),
+                      'spanning_cluster': rng.rand(),
+                      'spanning_cluster_ci': rng.rand(2),
+                      'moments': rng.rand(5), 'moments_ci': rng.rand(5, 2)})
+    out = percolate.microcanonical_averages_arrays(iter(steps))
+    assert out['N'] == N and out['M'] == M
+    for n, st in enumerate(steps):
+        assert out['max_cluster_size'][n] == st['max_cluster_size'] / N
+        assert (out['max_cluster_size_ci'][n] == st['max_cluster_size_ci'] / N).all()
+        assert out['spanning_cluster'][n] == st['spanning_cluster']
+        assert (out['spanning_cluster_ci'][n] == st['spanning_cluster_ci']).all()
+        assert (out['moments'][:, n] == st['moments'] / N).all()
+        assert (out['moments_ci'][:, n] == st['moments_ci'] / N).all()
+    assert out['moments'].shape == (5, M + 1)
+    assert out['moments_ci'].shape == (5, M + 1, 2)
+    with pytest.raises(AssertionError):
+        percolate.microcanonical_averages_arrays(iter(steps[1:]))

@@ -145,6 +145,24 @@ def test_statistics_end_to_end(gpu):
         np.testing.assert_allclose(mine[key], ref[key], rtol=1e-11, atol=1e-14)
 
 
+@pytest.mark.parametrize('spanning', [True, False])
+def test_statistics_equals_three_calls(gpu, spanning):
+    """statistics keeps the microcanonical statistics on the device; its
+    results equal microcanonical_averages -> microcanonical_averages_arrays
+    -> canonical_averages bit for bit (same global random stream)."""
+    g = percolate.spanning_2d_grid(10)
+    ps = np.linspace(0.2, 0.8, 33)
+    np.random.seed(11)
+    mine = percolate.statistics(g, ps, runs=25, spanning_cluster=spanning)
+    np.random.seed(11)
+    arrays = percolate.microcanonical_averages_arrays(
+        percolate.microcanonical_averages(g, runs=25, spanning_cluster=spanning))
+    ref = percolate.canonical_averages(ps, arrays)
+    assert sorted(mine) == sorted(ref)
+    for key in ref:
+        np.testing.assert_array_equal(np.asarray(mine[key]), np.asarray(ref[key]))
+
+
 def test_single_run_arrays(gpu, golden):
     kat = golden('kat_3x3.json')
     np.random.seed(42)

@@ -167,3 +167,23 @@ def test_span_on_first_merge(gpu):
     s1, s2 = host_sums(*eng.microcanonical_sums(res))
     o1, o2 = oracle.microcanonical_sums(expect)
     assert (s1 == o1).all() and (s2 == o2).all()
+
+
+def test_single_run_arrays_moments_past_2_64(gpu):
+    """percolate.single_run_arrays reports the run's exact moments rounded
+    to float64, also where they pass 2**64 (a chain of 400000 nodes near
+    its end: a few clusters of ~10^5 nodes), against the oracle's exact
+    128-bit moments of the same RandomState draw."""
+    import networkx as nx
+    from pypercolate_amd import percolate
+    n = 400000
+    np.random.seed(3)
+    got = percolate.single_run_arrays(graph=nx.path_graph(n), spanning_cluster=False)
+    np.random.seed(3)
+    perm = np.random.permutation(n - 1)
+    tr = oracle.bond_runs(chain_graph(n), perm[None, :], exact=True)
+    exact = oracle.exact_moments(tr)[0]                      # (S, 5) ints
+    np.testing.assert_array_equal(got['max_cluster_size'], tr['max_cluster_size'][0])
+    want = np.array([[float(v) for v in row] for row in exact.T])
+    np.testing.assert_allclose(got['moments'], want, rtol=2.3e-16, atol=0)
+    assert max(int(v) for v in exact[:, 4]) >= 1 << 64

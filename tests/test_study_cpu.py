@@ -31,3 +31,17 @@ def test_results_round_trip(tmp_path):
     for L in res:
         assert back[L].dtype == dtype
         np.testing.assert_array_equal(back[L], res[L])
+
+
+def test_checkpoint_names_depend_on_the_parameters():
+    """A rerun with other p, seeds, task split or spanning flag writes and
+    reads other checkpoint files (no stale task is reduced in)."""
+    ps = np.linspace(0.4, 0.6, 5)
+    seeds = np.arange(20)
+    base = study.task_key(ps, seeds, 4, True)
+    assert base == study.task_key(ps.copy(), seeds.copy(), 4, True)
+    others = {study.task_key(ps[:4], seeds, 4, True),
+              study.task_key(ps, seeds + 1, 4, True),
+              study.task_key(ps, seeds, 5, True),
+              study.task_key(ps, seeds, 4, False)}
+    assert base not in others and len(others) == 4
