@@ -58,15 +58,17 @@ int pc_record_bytes(uint32_t num_nodes);
  * Device permutations (performance mode).  Fills d_perm[r * perm_stride + i]
  * for r < num_runs, i < num_edges with a uniform random permutation of
  * 0..num_edges-1 for run first_run + r, keyed by (seed, run): a
- * Rao-Sandelius shuffle on Philox4x32-10 words (elements grouped into
- * buckets of about 4096 by random labels in a stable counting sort, each
- * bucket shuffled by a second level of labels and Fisher-Yates with
- * unbiased bounded draws; exactly uniform given uniform words).  Replaces
- * the host-side numpy RandomState(seed).permutation(num_edges) of
- * percolate/hpc.py:195,206 when bit-exactness with numpy is not required.
- * d_workspace holds pc_permute_philox_workspace_size bytes (per-run bucket
- * counts).  num_edges <= 2^23 and first_run + num_runs <= 2^32, else
- * PC_EOVERFLOW.
+ * Rao-Sandelius shuffle on Philox4x32-10 words: runs of more than 32768
+ * edges are first grouped into buckets of 2048-4096 by random labels in a
+ * stable counting sort; every bucket (the whole run up to 32768 edges) is
+ * shuffled in LDS by a second level of labels into sub-buckets of about 64
+ * and Fisher-Yates with unbiased bounded draws.  Exactly uniform given
+ * uniform words.  Replaces the host-side numpy
+ * RandomState(seed).permutation(num_edges) of percolate/hpc.py:195,206 when
+ * bit-exactness with numpy is not required.  d_workspace holds
+ * pc_permute_philox_workspace_size bytes (per-run bucket counts; none, and
+ * d_workspace may be NULL, up to 32768 edges).  num_edges <= 2^23 and
+ * first_run + num_runs <= 2^32, else PC_EOVERFLOW.
  */
 int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
                       uint32_t num_edges, int32_t *d_perm, int64_t perm_stride,

@@ -17,15 +17,22 @@
  * fraction = the order within the class).
  *
  * Layout (the device kernels follow it bit for bit):
- *   - 2**lb1 buckets, lb1 the least >= 0 with M <= bucket << lb1 (bucket
- *     4096 on the device); label of element i = the top lb1 bits of
- *     key(1, 0, i) (0 when lb1 = 0).  Elements grouped by label, stable.
+ *   - M <= whole (32768 on the device): one bucket, the whole run
+ *     (lb1 = 0), never through the wide path below.  Otherwise 2**lb1
+ *     buckets, lb1 the least with M <= bucket << lb1 (bucket 4096 on the
+ *     device); label of element i = the top lb1 bits of key(1, 0, i).
+ *     Elements grouped by label, stable.
  *   - Every bucket b of n elements, in place: n > group_cap (8192 on the
- *     device): plain Fisher-Yates with 32-bit draws from ctr
- *     (u, run, b, 4 << 28).  Otherwise 2**lb2 sub-buckets, lb2 the least
- *     <= 8 with n <= 32 << lb2; label of position j = the top lb2 bits of
+ *     device, in runs of several buckets): plain Fisher-Yates with 32-bit
+ *     draws from ctr (u, run, b, 4 << 28).
+ *     Otherwise 2**lb2 sub-buckets, lb2 the least with n <= sub << lb2
+ *     (sub 64 on the device); label of position j = the top lb2 bits of
  *     key(2, b, j); stable grouping, then Fisher-Yates of every sub-bucket
- *     s (start st, size c) with 16-bit draws from ctr (st, run, b, 3 << 28 | u).
+ *     (start st, size c) with 16-bit draws: the draws take the halves of
+ *     ctr (st, run, b, 3 << 28 | u), u = 0, 1, ... in turn, and a rejected
+ *     word is replaced by the next half of the sub-bucket's second stream,
+ *     ctr (st, run, b, 5 << 28 | v), v = 0, 1, ... (so every accepted draw
+ *     takes exactly one word of the first stream, whatever was rejected).
  *   - key(tag, b, j) = 16-bit half (j >> 6) & 7 of
  *     Philox(ctr ((j >> 9) << 6 | (j & 63), run, b, tag << 28)), the halves
  *     of a word taken low then high.
@@ -87,35 +94,32 @@ static uint32_t key16(const struct shuffle_ctx *x, uint32_t tag, uint32_t b,
     return half16(w, (uint32_t)((j >> 6) & 7u));
 }
 
-/* stream of 16-bit draws of one sub-bucket */
+/* the two streams of 16-bit words of one sub-bucket: ctr[3] = tag << 28 | u */
 struct draw16 {
     uint32_t ctr[4];
-    uint32_t w[4];
-    uint32_t h;
-    uint32_t u;
+    uint32_t w[4], h, u;   /* first stream */
+    uint32_t wr[4], hr, v; /* replacement words */
 };
 
-static uint32_t next16(const struct shuffle_ctx *x, struct draw16 *d)
+static uint32_t next16(const struct shuffle_ctx *x, uint32_t *ctr, uint32_t tag,
+                       uint32_t *w, uint32_t *h, uint32_t *u)
 {
-    if (d->h == 8u) {
-        d->ctr[3] = (3u << 28) | d->u;
-        nzo_philox4x32_10(d->ctr, x->key, d->w);
-        d->u++;
-        d->h = 0;
+    if (*h == 8u) {
+        ctr[3] = (tag << 28) | *u;
+        nzo_philox4x32_10(ctr, x->key, w);
+        ++*u;
+        *h = 0;
     }
-    return half16(d->w, d->h++);
+    return half16(w, (*h)++);
 }
 
 static uint32_t bounded16(const struct shuffle_ctx *x, struct draw16 *d,
                           uint32_t r)
 {
-    for (;;) {
-        const uint32_t m = next16(x, d) * r;
-        const uint32_t lo = m & 0xFFFFu;
-        if (lo < r && lo < (65536u - r) % r)
-            continue;
-        return m >> 16;
-    }
+    uint32_t m = next16(x, d->ctr, 3u, d->w, &d->h, &d->u) * r;
+    while ((m & 0xFFFFu) < r && (m & 0xFFFFu) < (65536u - r) % r)
+        m = next16(x, d->ctr, 5u, d->wr, &d->hr, &d->v) * r;
+    return m >> 16;
 }
 
 static void swap32(int32_t *a, int64_t i, int64_t j)
@@ -153,7 +157,7 @@ static void shuffle_wide(const struct shuffle_ctx *x, int32_t *a, int64_t n,
 }
 
 static void shuffle_group(const struct shuffle_ctx *x, int32_t *a, int64_t n,
-                          uint32_t b, int64_t cap)
+                          uint32_t b, int64_t cap, int64_t sub)
 {
     if (n <= 1)
         return;
@@ -162,7 +166,7 @@ static void shuffle_group(const struct shuffle_ctx *x, int32_t *a, int64_t n,
         return;
     }
     uint32_t lb = 0;
-    while ((32ll << lb) < n && lb < 8)
+    while ((sub << lb) < n)
         ++lb;
     const uint32_t B = 1u << lb;
     uint32_t *lab = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
@@ -185,7 +189,8 @@ static void shuffle_group(const struct shuffle_ctx *x, int32_t *a, int64_t n,
         const int64_t st = start[s], c = start[s + 1] - st;
         if (c < 2)
             continue;
-        struct draw16 d = {{(uint32_t)st, x->run, b, 0}, {0, 0, 0, 0}, 8u, 0u};
+        struct draw16 d = {{(uint32_t)st, x->run, b, 0}, {0, 0, 0, 0}, 8u, 0u,
+                           {0, 0, 0, 0}, 8u, 0u};
         for (int64_t j = c - 1; j >= 1; --j)
             swap32(tmp + st, j, bounded16(x, &d, (uint32_t)(j + 1)));
     }
@@ -197,23 +202,29 @@ static void shuffle_group(const struct shuffle_ctx *x, int32_t *a, int64_t n,
 
 /*
  * The permutation of run `run` for num_edges edges under `seed`, into out.
- * bucket and group_cap are the layout's size parameters (4096 and 8192 on
- * the device; tests shrink them to reach the wide Fisher-Yates path and
- * many buckets on small inputs).  Returns 0, or -1 for sizes outside the
- * layout (more than 2**23 edges, or more than 2**16 buckets).
+ * bucket, group_cap, whole and sub are the layout's size parameters (4096,
+ * 8192, 32768 and 64 on the device; tests shrink them to reach many
+ * buckets, the wide Fisher-Yates path and sub-bucket labels on small
+ * inputs).  Returns 0, or -1 for sizes outside the layout (more than 2**23
+ * edges, more than 2**16 buckets or sub-buckets, or a group_cap or whole
+ * above 2**16, where a sub-bucket could outgrow its 16-bit draws).
  */
 int nzo_bucket_permutation(uint64_t seed, uint32_t run, int64_t num_edges,
-                           int64_t bucket, int64_t group_cap, int32_t *out)
+                           int64_t bucket, int64_t group_cap, int64_t whole,
+                           int64_t sub, int32_t *out)
 {
     struct shuffle_ctx x = {{(uint32_t)seed, (uint32_t)(seed >> 32)}, run};
     const int64_t M = num_edges;
-    if (M < 0 || M > (1ll << 23))
+    if (M < 0 || M > (1ll << 23) || bucket < 1 || sub < 1 || group_cap > 65536 ||
+        whole > 65536)
         return -1;
     uint32_t lb1 = 0;
-    while ((bucket << lb1) < M)
-        ++lb1;
+    if (M > whole)
+        while ((bucket << lb1) < M)
+            ++lb1;
     if (lb1 > 16)
         return -1;
+    const int64_t cap = lb1 ? group_cap : whole;
     const uint32_t B = 1u << lb1;
     uint32_t *lab = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(M ? M : 1));
     int64_t *start = (int64_t *)calloc((size_t)B + 1, sizeof(int64_t));
@@ -227,24 +238,32 @@ int nzo_bucket_permutation(uint64_t seed, uint32_t run, int64_t num_edges,
     memcpy(fill, start, sizeof(int64_t) * B);
     for (int64_t i = 0; i < M; ++i)
         out[fill[lab[i]]++] = (int32_t)i;
-    for (uint32_t b = 0; b < B; ++b)
-        shuffle_group(&x, out + start[b], start[b + 1] - start[b], b, group_cap);
+    int rc = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        const int64_t n = start[b + 1] - start[b];
+        if (n <= cap && n > (sub << 16))
+            rc = -1;
+        else
+            shuffle_group(&x, out + start[b], n, b, cap, sub);
+    }
     free(fill);
     free(lab);
     free(start);
-    return 0;
+    return rc;
 }
 
 /* Many runs (first_run + r, r < runs), out [runs][num_edges]; OpenMP over
  * runs. */
 int nzo_bucket_permutations(uint64_t seed, uint32_t first_run, int64_t runs,
                             int64_t num_edges, int64_t bucket,
-                            int64_t group_cap, int32_t *out, int threads)
+                            int64_t group_cap, int64_t whole, int64_t sub,
+                            int32_t *out, int threads)
 {
     int rc = 0;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads) reduction(| : rc)
     for (int64_t r = 0; r < runs; ++r)
         rc |= nzo_bucket_permutation(seed, first_run + (uint32_t)r, num_edges,
-                                     bucket, group_cap, out + r * num_edges);
+                                     bucket, group_cap, whole, sub,
+                                     out + r * num_edges);
     return rc;
 }

@@ -76,7 +76,7 @@ def _load():
         lib.nzo_philox4x32_10.argtypes = [vp, vp, vp]
         lib.nzo_bucket_permutations.restype = ctypes.c_int
         lib.nzo_bucket_permutations.argtypes = [ctypes.c_uint64, ctypes.c_uint32,
-                                                i64, i64, i64, i64, vp,
+                                                i64, i64, i64, i64, i64, i64, vp,
                                                 ctypes.c_int]
         lib.nzo_bond_sums.restype = ctypes.c_int
         lib.nzo_bond_sums.argtypes = [i64, i64, i64, vp, ctypes.c_int, i64,
@@ -511,6 +511,8 @@ def philox4x32_10(ctr, key):
 # The size parameters of the device's layout (pc_permute_philox).
 SHUFFLE_BUCKET = 4096
 SHUFFLE_GROUP_CAP = 8192
+SHUFFLE_WHOLE = 32768
+SHUFFLE_SUB = 64
 
 
 def philox4x32_10_c(ctr, key):
@@ -526,7 +528,7 @@ def philox4x32_10_c(ctr, key):
 
 def bucket_permutations(seed, first_run, runs, num_edges,
                         bucket=SHUFFLE_BUCKET, group_cap=SHUFFLE_GROUP_CAP,
-                        threads=None):
+                        whole=SHUFFLE_WHOLE, sub=SHUFFLE_SUB, threads=None):
     """The permutations pc_permute_philox draws for runs first_run ..
     first_run + runs - 1 (int32 array (runs, M)), restated sequentially in
     C (nz_shuffle.c): a Philox-keyed Rao-Sandelius shuffle.  The size
@@ -537,7 +539,7 @@ def bucket_permutations(seed, first_run, runs, num_edges,
     out = np.zeros((int(runs), M), dtype=np.int32)
     rc = lib.nzo_bucket_permutations(
         int(seed) & 0xFFFFFFFFFFFFFFFF, int(first_run), int(runs), M,
-        int(bucket), int(group_cap), _p(out),
+        int(bucket), int(group_cap), int(whole), int(sub), _p(out),
         int(threads or os.cpu_count() or 1))
     if rc:
         raise ValueError('oracle: size outside the shuffle layout')
@@ -549,10 +551,16 @@ def _half16(words, h):
 
 
 def bucket_permutation_py(seed, run, num_edges, bucket=SHUFFLE_BUCKET,
-                          group_cap=SHUFFLE_GROUP_CAP):
+                          group_cap=SHUFFLE_GROUP_CAP, whole=SHUFFLE_WHOLE,
+                          sub=SHUFFLE_SUB, stats=None):
     """The same layout as bucket_permutations, written again in pure Python
     on top of philox4x32_10 (pinned by the Random123 vectors), for small
-    inputs: it pins the C restatement's counters, labels and draws."""
+    inputs: it pins the C restatement's counters, labels and draws.
+    ``stats`` (a dict), if given, counts the rejected 16-bit words
+    (``'rejected'``) and the wide Fisher-Yates buckets (``'wide'``)."""
+    stats = {} if stats is None else stats
+    stats.setdefault('rejected', 0)
+    stats.setdefault('wide', 0)
     key = (seed & MASK32, (seed >> 32) & MASK32)
 
     def key16(tag, b, j):
@@ -565,11 +573,12 @@ def bucket_permutation_py(seed, run, num_edges, bucket=SHUFFLE_BUCKET,
             groups[lab].append(it)
         return groups
 
-    def shuffle_bucket(a, b):
+    def shuffle_bucket(a, b, cap):
         n = len(a)
         if n <= 1:
             return list(a)
-        if n > group_cap:
+        if n > cap:
+            stats['wide'] += 1
             a = list(a)
             words, u = [], 0
             for j in range(n - 1, 0, -1):
@@ -587,26 +596,29 @@ def bucket_permutation_py(seed, run, num_edges, bucket=SHUFFLE_BUCKET,
                 a[j], a[k] = a[k], a[j]
             return a
         lb = 0
-        while (32 << lb) < n and lb < 8:
+        while (sub << lb) < n:
             lb += 1
         labels = [key16(2, b, j) >> (16 - lb) if lb else 0 for j in range(n)]
         groups = stable_group(a, labels, 1 << lb)
         out, st = [], 0
         for g in groups:
             c = len(g)
-            halves, u = [], 0
+            main, spare = [], []       # the sub-bucket's two word streams
+            u = v = 0
             for j in range(c - 1, 0, -1):
                 r = j + 1
-                while True:
-                    if not halves:
-                        w = philox4x32_10((st, run, b, (3 << 28) | u), key)
-                        halves = [_half16(w, h) for h in range(8)]
-                        u += 1
-                    m = halves.pop(0) * r
-                    lo = m & 0xFFFF
-                    if lo < r and lo < (65536 - r) % r:
-                        continue
-                    break
+                if not main:
+                    w = philox4x32_10((st, run, b, (3 << 28) | u), key)
+                    main = [_half16(w, h) for h in range(8)]
+                    u += 1
+                m = main.pop(0) * r
+                while (m & 0xFFFF) < r and (m & 0xFFFF) < (65536 - r) % r:
+                    stats['rejected'] += 1
+                    if not spare:
+                        w = philox4x32_10((st, run, b, (5 << 28) | v), key)
+                        spare = [_half16(w, h) for h in range(8)]
+                        v += 1
+                    m = spare.pop(0) * r
                 k = m >> 16
                 g[j], g[k] = g[k], g[j]
             out.extend(g)
@@ -615,11 +627,11 @@ def bucket_permutation_py(seed, run, num_edges, bucket=SHUFFLE_BUCKET,
 
     M = int(num_edges)
     lb1 = 0
-    while (bucket << lb1) < M:
+    while M > whole and (bucket << lb1) < M:
         lb1 += 1
     labels = [key16(1, 0, i) >> (16 - lb1) if lb1 else 0 for i in range(M)]
     buckets = stable_group(list(range(M)), labels, 1 << lb1)
     out = []
     for b, items in enumerate(buckets):
-        out.extend(shuffle_bucket(items, b))
+        out.extend(shuffle_bucket(items, b, group_cap if lb1 else whole))
     return np.array(out, dtype=np.int64)

@@ -225,37 +225,40 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0,
 //
 // Layout (the oracle's nz_shuffle.c restates it; perm[i] = edge index added
 // at step i + 1):
-//   - 2^lb1 buckets, lb1 the least with M <= SHUF_BUCKET << lb1; label of
-//     element i = the top lb1 bits of key(1, 0, i) (none when lb1 = 0).
-//   - Bucket b of n elements, in their grouped (increasing) order: n >
-//     SHUF_GROUP_CAP (odds below e^-1500 at these sizes): plain Fisher-Yates
-//     in HBM with 32-bit draws from ctr (u, run, b, 4 << 28).  Otherwise
-//     2^lb2 sub-buckets (the least lb2 <= 8 with n <= 32 << lb2), label of
+//   - M <= SHUF_WHOLE: the whole run is one bucket (lb1 = 0), shuffled by
+//     one workgroup straight from the identity.  Otherwise 2^lb1 buckets,
+//     lb1 the least with M <= SHUF_BUCKET << lb1; label of element i = the
+//     top lb1 bits of key(1, 0, i).
+//   - Bucket b of n elements, in their grouped (increasing) order: in runs
+//     of several buckets, n > SHUF_GROUP_CAP (odds below e^-1500 at these
+//     sizes): plain Fisher-Yates in HBM with 32-bit draws from ctr
+//     (u, run, b, 4 << 28).  Otherwise
+//     2^lb2 sub-buckets (the least lb2 with n <= SHUF_SUB << lb2), label of
 //     position j = the top lb2 bits of key(2, b, j), stable grouping, then
-//     Fisher-Yates of each sub-bucket (start st, size c) with 16-bit draws
-//     from ctr (st, run, b, 3 << 28 | u).
+//     Fisher-Yates of each sub-bucket (start st, size c) with 16-bit draws:
+//     the halves of ctr (st, run, b, 3 << 28 | u) in turn, a rejected word
+//     replaced by the next half of ctr (st, run, b, 5 << 28 | v).  Every
+//     accepted draw so takes exactly one word of the first stream, and the
+//     lanes of a wave, which start their sub-buckets together, refill it
+//     together.
 //   - key(tag, b, j) = 16-bit half (j >> 6) & 7 (low half first) of
 //     Philox(ctr ((j >> 9) << 6 | (j & 63), run, b, tag << 28)): one Philox
 //     call per lane labels 8 positions 64 apart, so a wave labels 512
 //     consecutive positions per call and visits them in order.
 //   - Bounded draw on [0, r) from a w-bit word x: m = x * r, rejected while
 //     (m mod 2^w) < (2^w - r) mod r (Lemire), result m >> w.
-// Kernels: count (labels per bucket and chunk of SHUF_CHUNK elements) ->
-// scan (per run) -> scatter (stable grouping into the output row) -> group
-// (one workgroup per bucket, in LDS).  Small workgroups with little LDS, so
-// many run per CU and hide the latencies of the sequential parts.
+// Kernels: for several buckets count (labels per bucket and chunk of
+// SHUF_CHUNK elements) -> scan (per run) -> scatter (stable grouping into
+// the output row); then group (one workgroup per bucket, in LDS).
 // ---------------------------------------------------------------------------
 #define SHUF_BUCKET 4096u
 #define SHUF_GROUP_CAP 8192u
+#define SHUF_WHOLE 32768u
+#define SHUF_SUB 64u
+#define SHUF_SMALL_CAP 4096u  // one-bucket runs up to this size: 2-wave groups
 #define SHUF_CHUNK 4096u
 #define SHUF_MAX_EDGES (1u << 23)
 #define SHUF_W 4
-#define SHUF_GW 4             // waves of a bucket's workgroup
-#define SHUF_HIST_WORDS 128u  // per wave: 256 16-bit sub-bucket counters
-// elements each thread of a bucket's workgroup gathers at most, and groups
-// of 512 positions each of its waves labels at most
-#define SHUF_GATHER (SHUF_GROUP_CAP / (64u * SHUF_GW))
-#define SHUF_GROUPS_PER_WAVE ((SHUF_GROUP_CAP / 512u + SHUF_GW - 1u) / SHUF_GW)
 
 __device__ __forceinline__ uint32_t half16(const u32x4 &w, uint32_t h)
 {
@@ -270,17 +273,20 @@ __device__ __forceinline__ u32x4 shuf_philox(uint32_t x, uint32_t run, uint32_t 
 }
 
 // Lanes (of the active ones in `valid`) whose label equals this lane's, for a
-// label of `bits` bits: one ballot per bit.
+// label of `bits` bits: one ballot per bit.  Per bit, s is all ones where
+// the lane's bit is set, and each 32-bit half of the mask keeps the lanes
+// whose ballot bit equals s (one three-input bit operation per half).
 __device__ __forceinline__ uint64_t match_label(uint32_t label, uint32_t bits,
                                                 uint64_t valid)
 {
-    uint64_t m = valid;
+    uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
     for (uint32_t i = 0; i < bits; ++i) {
-        const bool set = (label >> i) & 1u;
-        const uint64_t ones = __ballot(set);
-        m &= set ? ones : ~ones;
+        const uint32_t s = (uint32_t)((int32_t)(label << (31u - i)) >> 31);
+        const uint64_t ones = __ballot(s != 0u);
+        lo &= ~((uint32_t)ones ^ s);
+        hi &= ~((uint32_t)(ones >> 32) ^ s);
     }
-    return m;
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -304,37 +310,52 @@ __device__ __forceinline__ uint32_t lemire_threshold32(uint32_t r)
     return (0u - r) % r;
 }
 
-// Fisher-Yates of a[0..c) with the 16-bit draw stream of sub-bucket start st.
-// The Philox words are kept as scalars: a struct indexed by the running half
-// counter would be demoted to a private (scratch or LDS) array.
+// A rejected 16-bit draw (m = x * r with m mod 2^16 < r) redrawn from the
+// sub-bucket's replacement stream until accepted; v counts the replacement
+// words taken.
+__device__ __forceinline__ uint32_t shuffle_redraw(uint32_t m, uint32_t r,
+                                                   uint32_t st, uint32_t run,
+                                                   uint32_t b, uint32_t k0,
+                                                   uint32_t k1, uint32_t &v)
+{
+    const uint32_t thr = lemire_threshold16(r);
+    while ((m & 0xFFFFu) < thr) {
+        const u32x4 w = shuf_philox(st, run, b, (5u << 28) | (v >> 3), k0, k1);
+        const uint32_t h = v & 7u;
+        const uint32_t wd = (h & 4u) ? ((h & 2u) ? w.w : w.z) : ((h & 2u) ? w.y : w.x);
+        m = ((wd >> ((h & 1u) * 16u)) & 0xFFFFu) * r;
+        ++v;
+    }
+    return m;
+}
+
+// Fisher-Yates of a[0..c) with the 16-bit draw streams of sub-bucket start
+// st.  The Philox words are kept as scalars: a struct indexed by the running
+// half counter would be demoted to a private (scratch or LDS) array.
 template <typename E>
 __device__ __forceinline__ void shuffle_sub(E *a, uint32_t c, uint32_t st,
                                             uint32_t run, uint32_t b, uint32_t k0,
                                             uint32_t k1)
 {
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-    uint32_t h = 8u, u = 0u;
+    uint32_t h = 8u, u = 0u, v = 0u;
     for (uint32_t j = c - 1u; j >= 1u; --j) {
         const uint32_t r = j + 1u;
-        uint32_t m;
-        for (;;) {
-            if (h == 8u) {
-                const u32x4 w = shuf_philox(st, run, b, (3u << 28) | u, k0, k1);
-                w0 = w.x;
-                w1 = w.y;
-                w2 = w.z;
-                w3 = w.w;
-                ++u;
-                h = 0u;
-            }
-            const uint32_t wd = (h & 4u) ? ((h & 2u) ? w3 : w2) : ((h & 2u) ? w1 : w0);
-            m = ((wd >> ((h & 1u) * 16u)) & 0xFFFFu) * r;
-            ++h;
-            const uint32_t lo = m & 0xFFFFu;
-            if (__builtin_expect(lo < r, 0) && lo < lemire_threshold16(r))
-                continue;
-            break;
+        if (h == 8u) {
+            const u32x4 w = shuf_philox(st, run, b, (3u << 28) | u, k0, k1);
+            w0 = w.x;
+            w1 = w.y;
+            w2 = w.z;
+            w3 = w.w;
+            ++u;
+            h = 0u;
         }
+        const uint32_t wd = (h & 4u) ? ((h & 2u) ? w3 : w2) : ((h & 2u) ? w1 : w0);
+        // 16-bit word times r <= 2^15 + 1: a 24-bit multiply is exact
+        uint32_t m = __umul24((wd >> ((h & 1u) * 16u)) & 0xFFFFu, r);
+        ++h;
+        if (__builtin_expect((m & 0xFFFFu) < r, 0))
+            m = shuffle_redraw(m, r, st, run, b, k0, k1, v);
         const uint32_t k = m >> 16;
         const E t = a[j];
         a[j] = a[k];
@@ -574,35 +595,48 @@ __global__ __launch_bounds__(64 * SHUF_W) void shuffle_scatter_kernel(
     }
 }
 
-// One bucket a[0..n), n <= SHUF_GROUP_CAP, shuffled by a workgroup of W
+// Words per wave of the sub-bucket counters (16-bit, in pairs) for groups of
+// at most cap elements.
+__host__ __device__ __forceinline__ uint32_t shuf_hist_words(uint32_t cap)
+{
+    uint32_t lb = 0;
+    while ((SHUF_SUB << lb) < cap)
+        ++lb;
+    return ((1u << lb) + 1u) / 2u;
+}
+
+// One bucket a[0..n), n <= the launch's cap, shuffled by a workgroup of W
 // waves: sub-bucket labels, stable grouping of the local indices 0..n-1 in
-// ent[] (16-bit), Fisher-Yates of each sub-bucket, then a[t] = old a[ent[t]]
-// (all loads in flight together, all stores after a barrier).  hist holds
-// W * SHUF_HIST_WORDS words (16-bit counters in pairs), wsum W.
-template <int W>
+// ent[] (16-bit), Fisher-Yates of each sub-bucket (one thread each), then
+// a[t] = old a[ent[t]] (all loads in flight together, all stores after a
+// barrier), or a[t] = ent[t] when a is the identity (IDENT: the whole run
+// is one bucket).  hist holds W * hw words, wsum W; GPW bounds the groups
+// of 512 positions a wave labels.
+template <int W, int GPW, bool IDENT>
 __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
                                   uint32_t k0, uint32_t k1, int32_t *a,
-                                  uint16_t *ent, uint32_t *hist, uint32_t *wsum)
+                                  uint16_t *ent, uint32_t *hist, uint32_t hw,
+                                  uint32_t *wsum)
 {
     const uint32_t T = 64u * W, tid = threadIdx.x, wave = tid >> 6,
                    lane = tid & 63u;
     uint32_t lb = 0;
-    while ((32u << lb) < n && lb < 8u)
+    while ((SHUF_SUB << lb) < n)
         ++lb;
     const uint32_t nb = 1u << lb;
     const uint32_t G = (n + 511u) / 512u;
     const uint32_t g0 = wave * G / W, g1 = (wave + 1u) * G / W;
-    uint32_t *my = hist + wave * SHUF_HIST_WORDS;
+    uint32_t *my = hist + wave * hw;
 
-    for (uint32_t i = tid; i < W * SHUF_HIST_WORDS; i += T)
+    for (uint32_t i = tid; i < W * hw; i += T)
         hist[i] = 0u;
     __syncthreads();
 
     // labels of this wave's positions (8 per Philox call, kept for the
-    // grouping below: at most SHUF_GROUPS_PER_WAVE groups), counted per wave
-    u32x4 keys[SHUF_GROUPS_PER_WAVE];
+    // grouping below), counted per wave
+    u32x4 keys[GPW];
 #pragma unroll
-    for (uint32_t gi = 0; gi < SHUF_GROUPS_PER_WAVE; ++gi) {
+    for (uint32_t gi = 0; gi < (uint32_t)GPW; ++gi) {
         const uint32_t g = g0 + gi;
         keys[gi] = (lb && g < g1)
                        ? shuf_philox((g << 6) | lane, run, b, 2u << 28, k0, k1)
@@ -629,7 +663,7 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
         uint32_t tot = 0;
         for (uint32_t p = p0; p < p1; ++p)
             for (uint32_t w = 0; w < (uint32_t)W; ++w) {
-                const uint32_t word = hist[w * SHUF_HIST_WORDS + p];
+                const uint32_t word = hist[w * hw + p];
                 tot += (word & 0xFFFFu) + (word >> 16);
             }
         const uint32_t inc = wave_scan_add(tot);
@@ -643,11 +677,11 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
             // sub-bucket 2p (low halves) of every wave, then 2p + 1 (high)
             uint32_t lo_total = 0;
             for (uint32_t w = 0; w < (uint32_t)W; ++w)
-                lo_total += hist[w * SHUF_HIST_WORDS + p] & 0xFFFFu;
+                lo_total += hist[w * hw + p] & 0xFFFFu;
             uint32_t a0 = run_, a1 = run_ + lo_total;
             for (uint32_t w = 0; w < (uint32_t)W; ++w) {
-                const uint32_t word = hist[w * SHUF_HIST_WORDS + p];
-                hist[w * SHUF_HIST_WORDS + p] = a0 | (a1 << 16);
+                const uint32_t word = hist[w * hw + p];
+                hist[w * hw + p] = a0 | (a1 << 16);
                 a0 += word & 0xFFFFu;
                 a1 += word >> 16;
             }
@@ -660,7 +694,7 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
     // rank among the lanes of the same label, the label's first lane
     // advances the wave's base
 #pragma unroll
-    for (uint32_t gi = 0; gi < SHUF_GROUPS_PER_WAVE; ++gi) {
+    for (uint32_t gi = 0; gi < (uint32_t)GPW; ++gi) {
         const uint32_t g = g0 + gi;
         if (g < g1) {
 #pragma unroll
@@ -685,7 +719,7 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
 
     // Fisher-Yates per sub-bucket: after the grouping the last wave's base
     // of sub-bucket s is the end of s
-    const uint32_t *last = hist + (W - 1) * SHUF_HIST_WORDS;
+    const uint32_t *last = hist + (W - 1) * hw;
     for (uint32_t s = tid; s < nb; s += T) {
         const uint32_t en = (last[s >> 1] >> ((s & 1u) * 16u)) & 0xFFFFu;
         const uint32_t st =
@@ -695,21 +729,27 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
     }
     __syncthreads();
 
+    if (IDENT) {
+        for (uint32_t t = tid; t < n; t += T)
+            a[t] = (int32_t)ent[t];
+        return;
+    }
     // a[t] = old a[ent[t]]: every load issued before any store of the
     // workgroup (the empty asm makes each thread wait for its loads before
     // the barrier)
-    int32_t v[SHUF_GATHER];
+    constexpr uint32_t GATHER = IDENT ? 1u : SHUF_GROUP_CAP / (64u * W);
+    int32_t v[GATHER];
 #pragma unroll
-    for (uint32_t i = 0; i < SHUF_GATHER; ++i) {
+    for (uint32_t i = 0; i < GATHER; ++i) {
         const uint32_t t = tid + i * T;
         v[i] = t < n ? a[ent[t]] : 0;
     }
 #pragma unroll
-    for (uint32_t i = 0; i < SHUF_GATHER; ++i)
+    for (uint32_t i = 0; i < GATHER; ++i)
         asm volatile("" ::"v"(v[i]));
     __syncthreads();
 #pragma unroll
-    for (uint32_t i = 0; i < SHUF_GATHER; ++i) {
+    for (uint32_t i = 0; i < GATHER; ++i) {
         const uint32_t t = tid + i * T;
         if (t < n)
             a[t] = v[i];
@@ -717,28 +757,47 @@ __device__ void shuffle_group_lds(uint32_t n, uint32_t b, uint32_t run,
 }
 
 // Group kernel: bucket b = blockIdx.x of run r0 + blockIdx.y, in place in
-// the output row.
-__global__ __launch_bounds__(64 * SHUF_GW) void shuffle_group_kernel(
+// the output row; cap bounds the buckets' sizes handled in LDS (the run's
+// size when IDENT: one bucket, the identity before the shuffle).  Dynamic
+// LDS: shuf_group_lds_bytes(W, cap).
+template <int W, bool IDENT>
+__global__ __launch_bounds__(64 * W) void shuffle_group_kernel(
     uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t lb1, uint32_t r0,
-    const uint32_t *__restrict__ starts, int32_t *__restrict__ perm,
-    int64_t stride)
+    uint32_t cap, const uint32_t *__restrict__ starts,
+    int32_t *__restrict__ perm, int64_t stride)
 {
-    __shared__ uint16_t ent[SHUF_GROUP_CAP];
-    __shared__ uint32_t hist[SHUF_GW * SHUF_HIST_WORDS];
-    __shared__ uint32_t wsum[SHUF_GW];
-    const uint32_t nb1 = 1u << lb1, b = blockIdx.x, r = r0 + blockIdx.y;
+    extern __shared__ uint32_t gdyn[];
+    constexpr uint32_t CAP =
+        IDENT ? (W == 2 ? SHUF_SMALL_CAP : SHUF_WHOLE) : SHUF_GROUP_CAP;
+    constexpr int GPW = (int)(CAP / (512u * W));
+    const uint32_t hw = shuf_hist_words(cap);
+    uint32_t *hist = gdyn, *wsum = gdyn + W * hw;
+    uint16_t *ent = (uint16_t *)(wsum + W);
+    const uint32_t b = blockIdx.x, r = r0 + blockIdx.y;
     const uint32_t run = first_run + r;
-    const uint32_t *st = starts + (uint64_t)r * (nb1 + 1u);
-    const uint32_t o = st[b], n = st[b + 1u] - o;
+    uint32_t o = 0, n = cap;
+    if (!IDENT) {
+        const uint32_t *st = starts + (uint64_t)r * ((1u << lb1) + 1u);
+        o = st[b];
+        n = st[b + 1u] - o;
+    }
     int32_t *a = perm + (int64_t)r * stride + o;
-    if (n <= 1u)
+    if (n <= 1u) {
+        if (IDENT && n == 1u && threadIdx.x == 0u)
+            a[0] = 0;
         return;
-    if (n > SHUF_GROUP_CAP) {
+    }
+    if (!IDENT && n > SHUF_GROUP_CAP) {
         if (threadIdx.x == 0u)
             shuffle_wide(a, n, run, b, k0, k1);
         return;
     }
-    shuffle_group_lds<SHUF_GW>(n, b, run, k0, k1, a, ent, hist, wsum);
+    shuffle_group_lds<W, GPW, IDENT>(n, b, run, k0, k1, a, ent, hist, hw, wsum);
+}
+
+static size_t shuf_group_lds_bytes(uint32_t W, uint32_t cap)
+{
+    return (size_t)(W * shuf_hist_words(cap) + W) * 4u + (((size_t)cap * 2u + 3u) & ~(size_t)3u);
 }
 
 // ---------------------------------------------------------------------------
@@ -3003,8 +3062,9 @@ extern "C" int pc_record_bytes(uint32_t num_nodes)
 static void shuffle_plan(uint32_t num_edges, uint32_t &lb1, uint32_t &chunks)
 {
     lb1 = 0;
-    while (((uint64_t)SHUF_BUCKET << lb1) < num_edges)
-        ++lb1;
+    if (num_edges > SHUF_WHOLE)
+        while (((uint64_t)SHUF_BUCKET << lb1) < num_edges)
+            ++lb1;
     chunks = (num_edges + SHUF_CHUNK - 1u) / SHUF_CHUNK;
 }
 
@@ -3020,7 +3080,8 @@ extern "C" int pc_permute_philox_workspace_size(uint32_t num_edges,
     uint32_t lb1 = 0, chunks = 0;
     shuffle_plan(num_edges, lb1, chunks);
     const uint64_t nb1 = 1ull << lb1;
-    *bytes = num_edges ? (uint64_t)num_runs * (nb1 * chunks + nb1 + 1u) * 4u : 0u;
+    // a run of one bucket is shuffled in LDS from the identity: no counts
+    *bytes = lb1 ? (uint64_t)num_runs * (nb1 * chunks + nb1 + 1u) * 4u : 0u;
     return PC_OK;
 }
 
@@ -3041,20 +3102,34 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
         return pc_fail(PC_EOVERFLOW, "pc_permute_philox: run indices reach 2^32");
     uint64_t need = 0;
     pc_permute_philox_workspace_size(num_edges, num_runs, &need);
-    if (!d_workspace || workspace_bytes < need)
+    if (need && (!d_workspace || workspace_bytes < need))
         return pc_fail(PC_ENOMEM, "pc_permute_philox: workspace needs %llu bytes",
                        (unsigned long long)need);
     uint32_t lb1 = 0, chunks = 0;
     shuffle_plan(num_edges, lb1, chunks);
     const uint32_t nb1 = 1u << lb1;
     uint32_t *counts = (uint32_t *)d_workspace;
-    uint32_t *starts = counts + (uint64_t)num_runs * nb1 * chunks;
+    uint32_t *starts = counts ? counts + (uint64_t)num_runs * nb1 * chunks : nullptr;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const uint32_t fr = (uint32_t)first_run;
     hipStream_t s = as_stream(stream);
     // runs in slabs of at most 65535 (the grid's y extent)
     for (uint32_t r0 = 0; r0 < num_runs; r0 += 65535u) {
         const uint32_t rn = min(num_runs - r0, 65535u);
+        if (lb1 == 0u && num_edges <= SHUF_SMALL_CAP) {
+            shuffle_group_kernel<2, true><<<dim3(1, rn), 128,
+                                            shuf_group_lds_bytes(2, num_edges), s>>>(
+                k0, k1, fr, 0u, r0, num_edges, nullptr, d_perm, perm_stride);
+            PC_LAUNCH_CHECK("shuffle_group_kernel");
+            continue;
+        }
+        if (lb1 == 0u) {
+            shuffle_group_kernel<8, true><<<dim3(1, rn), 512,
+                                            shuf_group_lds_bytes(8, num_edges), s>>>(
+                k0, k1, fr, 0u, r0, num_edges, nullptr, d_perm, perm_stride);
+            PC_LAUNCH_CHECK("shuffle_group_kernel");
+            continue;
+        }
         shuffle_count_kernel<<<dim3(chunks, rn), 64 * SHUF_W, nb1 * 4u, s>>>(
             k0, k1, fr, num_edges, lb1, chunks, r0, counts);
         PC_LAUNCH_CHECK("shuffle_count_kernel");
@@ -3065,8 +3140,9 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
                                  (SHUF_W + 2u) * nb1 * 4u, s>>>(
             k0, k1, fr, num_edges, lb1, chunks, r0, counts, d_perm, perm_stride);
         PC_LAUNCH_CHECK("shuffle_scatter_kernel");
-        shuffle_group_kernel<<<dim3(nb1, rn), 64 * SHUF_GW, 0, s>>>(
-            k0, k1, fr, lb1, r0, starts, d_perm, perm_stride);
+        shuffle_group_kernel<4, false><<<dim3(nb1, rn), 256,
+                                         shuf_group_lds_bytes(4, SHUF_GROUP_CAP), s>>>(
+            k0, k1, fr, lb1, r0, SHUF_GROUP_CAP, starts, d_perm, perm_stride);
         PC_LAUNCH_CHECK("shuffle_group_kernel");
     }
     return PC_OK;

@@ -127,14 +127,15 @@ def test_global_state(gpu, length):
     _check_sums(eng, res, expect)
 
 
-@pytest.mark.parametrize('M', [1, 2, 12, 33, 480, 4096, 4097, 32512, 32769,
-                               65536, 65537, 130560, 2095104])
+@pytest.mark.parametrize('M', [1, 2, 12, 33, 64, 65, 480, 4096, 4097, 32512,
+                               32768, 32769, 65536, 65537, 130560, 2095104])
 def test_device_permutation_matches_oracle(gpu, M):
     """pc_permute_philox equals the oracle's restatement of the bucket
-    shuffle bit for bit: one bucket (M <= 4096), several (the L = 128
-    lattice: 8), 16- and 32-bit LDS entries (either side of 65536 edges),
-    several chunks of the count and scatter passes (the L = 256 and L = 1024
-    lattices)."""
+    shuffle bit for bit: the whole run as one bucket in the 2-wave (M <=
+    4096) and the 8-wave group kernel (up to 32768: the L = 128 lattice),
+    one sub-bucket or several, then several buckets through count, scan and
+    scatter (from 32769 edges; several chunks of those passes at the
+    L = 256 and L = 1024 lattices)."""
     eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
         [np.arange(M), np.arange(1, M + 1)], axis=1)))
     runs = 3 if M < 1000000 else 2

@@ -13,7 +13,7 @@ percolate/hpc.py:195,206) and checks that its permutations are uniform.
   square), in every form; the parity of the permutation (a keyed Feistel
   network on a power-of-two domain only makes even permutations); the
   position of single elements and the order of neighbours, with one bucket
-  and with many (M = 40000: ten buckets).
+  and with many (M = 40000: sixteen buckets).
 """
 import math
 
@@ -32,11 +32,16 @@ KAT = [
      (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
 ]
 
-DEVICE = (oracle.SHUFFLE_BUCKET, oracle.SHUFFLE_GROUP_CAP)
-# shrunk size parameters: (bucket, group_cap)
-MANY = (16, 1 << 20)   # many buckets, every one through sub-buckets
-WIDE = (2, 2)          # buckets above 2 through the wide Fisher-Yates path
-MIXED = (4, 5)         # buckets of both kinds
+DEVICE = (oracle.SHUFFLE_BUCKET, oracle.SHUFFLE_GROUP_CAP,
+          oracle.SHUFFLE_WHOLE, oracle.SHUFFLE_SUB)
+# shrunk size parameters: (bucket, group_cap, whole, sub)
+MANY = (16, 64, 64, 4)      # many buckets, each through sub-buckets of ~4
+WIDE = (2, 2, 2, 64)        # buckets above 2 through the wide Fisher-Yates path
+MIXED = (4, 5, 5, 2)        # buckets of both kinds
+SUBS = (65536, 65536, 65536, 1)      # one bucket, sub-buckets of ~1 element
+ONE = (65536, 65536, 65536, 65536)   # one bucket, one sub-bucket: long runs
+SPLIT = (4, 8, 100, 4)      # whole runs up to 100 (above the group cap), then
+                            # buckets of ~4 with the wide path above 8
 
 
 @pytest.mark.parametrize('ctr,key,expect', KAT)
@@ -45,7 +50,8 @@ def test_philox_kat(ctr, key, expect):
     assert oracle.philox4x32_10_c(ctr, key) == expect
 
 
-@pytest.mark.parametrize('params', [DEVICE, MANY, WIDE, MIXED, (40, 20)])
+@pytest.mark.parametrize('params', [DEVICE, MANY, WIDE, MIXED, SUBS, ONE,
+                                    SPLIT, (40, 20, 20, 8)])
 @pytest.mark.parametrize('M', [1, 2, 3, 7, 33, 64, 65, 300, 1000, 5000])
 def test_c_restatement_equals_python(M, params):
     got = oracle.bucket_permutations(0x123456789, 7, 3, M, *params)
@@ -55,8 +61,27 @@ def test_c_restatement_equals_python(M, params):
         assert sorted(got[r].tolist()) == list(range(M))
 
 
+def test_restatements_reach_rejections_and_wide_buckets():
+    """The cases above take the replacement stream of rejected 16-bit words
+    and the wide Fisher-Yates path, so the C-Python equality covers both."""
+    st = {}
+    for r in range(3):
+        oracle.bucket_permutation_py(0x123456789, 7 + r, 5000, *ONE, stats=st)
+    assert st['rejected'] > 20 and st['wide'] == 0
+    st = {}
+    oracle.bucket_permutation_py(0x123456789, 7, 1000, *WIDE, stats=st)
+    assert st['wide'] > 50
+
+
+def test_sizes_outside_the_layout_are_refused():
+    with pytest.raises(ValueError):
+        oracle.bucket_permutations(1, 0, 1, 10, 16, 1 << 17, 64, 64)
+    with pytest.raises(ValueError):
+        oracle.bucket_permutations(1, 0, 1, 10, 16, 64, 1 << 17, 64)
+
+
 def test_device_sizes_are_permutations_and_runs_differ():
-    for M in (4096, 4097, 32512, 40000, 130560):
+    for M in (4096, 4097, 8192, 8193, 32512, 32768, 32769, 40000, 130560):
         p = oracle.bucket_permutations(1234, 0, 3, M)
         for r in range(3):
             assert np.array_equal(np.sort(p[r]), np.arange(M))
@@ -75,7 +100,8 @@ def _lehmer_rank(perms):
     return rank
 
 
-@pytest.mark.parametrize('params', [DEVICE, WIDE, MIXED, (1, 8)])
+@pytest.mark.parametrize('params', [DEVICE, WIDE, MIXED, SUBS, (1, 8, 8, 2),
+                                    (1, 2, 3, 1)])
 @pytest.mark.parametrize('M', [2, 3, 4, 5])
 def test_every_ordering_equally_likely(M, params):
     """All M! orderings of a small list, over 400 * M! runs: chi square of
@@ -115,7 +141,8 @@ def test_parity_is_balanced(M):
 
 
 @pytest.mark.parametrize('M,params', [(1000, DEVICE), (40000, DEVICE),
-                                      (3000, MANY)])
+                                      (3000, MANY), (3000, SUBS),
+                                      (3000, SPLIT)])
 def test_positions_uniform(M, params):
     """Where single elements land (20 equal bins, chi square) and whether a
     neighbour pair keeps its order (binomial), over many runs; elements at
