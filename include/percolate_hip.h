@@ -75,6 +75,52 @@ int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
                       void *d_workspace, uint64_t workspace_bytes,
                       void *stream);
 
+/*
+ * One graph of a persistent multi-graph sweep (pc_sweep_multi): the
+ * arguments pc_sweep takes for one graph, with chunk states always written.
+ */
+typedef struct pc_sweep_graph {
+    uint32_t num_nodes, num_edges, num_runs;
+    int32_t span_on_first_merge;
+    uint32_t merge_rounds;          /* 0: the library default */
+    uint32_t reserved;
+    const int32_t *d_edges;         /* [num_edges][2] */
+    const uint8_t *d_node_sides;    /* [num_nodes] spanning sides, or NULL */
+    const int32_t *d_perm;          /* [num_runs][perm_stride] */
+    int64_t perm_stride;
+    void *d_records;                /* uint32 (num_nodes <= 65536) or uint64 */
+    int64_t rec_stride;
+    uint32_t *d_span_step;          /* [num_runs] */
+    uint64_t *d_chunk_state;        /* [num_runs][ceil(num_edges / 64)][8] */
+} pc_sweep_graph;
+
+/*
+ * Persistent sweep of several graphs in one launch: the per-system-size loop
+ * of the reference's finite-size study (percolate/share/jugfile.py:178-253,
+ * each size's runs through percolate/hpc.py:249-307) as one device work
+ * queue.  Every run of every graph is a queue item, numbered in the order of
+ * graphs[] (list the largest graphs first: their runs are the longest); each
+ * resident workgroup takes the next item with an atomic on a queue head,
+ * sweeps it (LDS or HBM union-find state by the graph's size) and repeats
+ * until the queue is empty.  Outputs per graph are pc_sweep's records, span
+ * steps and chunk states.
+ * pc_sweep_multi_workspace_size gives the bytes of the device table (graph
+ * descriptors and the queue head) and of the HBM-state workspace (one slot
+ * per resident workgroup); pc_sweep_multi_prepare writes the table (a
+ * synchronous copy, once per set of buffers); pc_sweep_multi clears the
+ * queue head with a kernel and launches the sweep (kernels only, so it may
+ * be captured in a HIP graph).  The table must be 256-byte aligned.
+ */
+int pc_sweep_multi_workspace_size(uint32_t num_graphs,
+                                  const pc_sweep_graph *graphs,
+                                  uint64_t *table_bytes,
+                                  uint64_t *workspace_bytes);
+int pc_sweep_multi_prepare(uint32_t num_graphs, const pc_sweep_graph *graphs,
+                           void *d_table, uint64_t table_bytes);
+int pc_sweep_multi(uint32_t num_graphs, const pc_sweep_graph *graphs,
+                   void *d_table, uint64_t table_bytes, void *d_workspace,
+                   uint64_t workspace_bytes, void *stream);
+
 /* Device workspace bytes pc_permute_philox needs. */
 int pc_permute_philox_workspace_size(uint32_t num_edges, uint32_t num_runs,
                                      uint64_t *bytes);
@@ -306,6 +352,41 @@ int pc_convert_f64_to_f32(const double *d_in, int64_t n, float *d_out,
 int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
                               const uint64_t *d_sum1, const uint64_t *d_sum2,
                               double *d_mean, double *d_std, void *stream);
+
+/*
+ * The reduction of canonical averages on the device: percolate/hpc.py:638-702
+ * (bond_reduce, through simoa.stats.online_variance), the pairwise merge of
+ * Chan, Golub & LeVeque of two sets of (n, mean, M2) for num_ps rows of
+ * num_columns statistics (element p * num_columns + c; the counts d_n_a[p],
+ * d_n_b[p] as doubles): mean = mean_a + delta * (n_b / n), M2 = M2_a + M2_b
+ * + delta * delta * (n_a * n_b / n), delta = mean_b - mean_a, n = n_a + n_b.
+ * d_mean_out and d_m2_out may be d_mean_a and d_m2_a.
+ */
+int pc_merge_moments(uint32_t num_ps, uint32_t num_columns, const double *d_n_a,
+                     const double *d_mean_a, const double *d_m2_a,
+                     const double *d_n_b, const double *d_mean_b,
+                     const double *d_m2_b, double *d_mean_out, double *d_m2_out,
+                     void *stream);
+
+/*
+ * Finalized canonical averages on the device: percolate/hpc.py:752-834
+ * (finalize_canonical_averages) for num_ps rows of num_columns statistics,
+ * element i = p * num_columns + c of d_mean and d_m2 (the mean and the sum
+ * of squared deviations over d_runs[p] runs).  d_out[k * P * C + i],
+ * k = 0..3: the mean and the sample standard deviation sqrt(M2 / (n - 1)),
+ * each divided by d_div[c] (the number of nodes for the percolation
+ * strength and the moments, 1 otherwise), and the Student-t interval
+ * bounds d_q_lo[p] * (std / sqrt(n)) + mean and d_q_hi[p] * ... + mean,
+ * each product and sum rounded separately as scipy.stats.t.interval does
+ * (the caller supplies t.ppf of (1 -+ (1 - alpha)) / 2 at n - 1 degrees of
+ * freedom); NaN where scipy's ppf refuses its arguments (n < 2, a scale
+ * that is not positive, a NaN mean).
+ */
+int pc_finalize_canonical(uint32_t num_ps, uint32_t num_columns,
+                          const uint32_t *d_runs, const double *d_mean,
+                          const double *d_m2, const double *d_div,
+                          const double *d_q_lo, const double *d_q_hi,
+                          double *d_out, void *stream);
 
 /*
  * The microcanonical statistics of every step with their confidence bounds,

@@ -660,6 +660,65 @@ def micro_ci(sum1, sum2, num_edges, num_runs, alpha):
     return out
 
 
+def t_interval_quantiles(runs, alpha):
+    """Per row, the Student-t quantiles scipy.stats.t.interval(1 - alpha,
+    df=runs - 1) multiplies the scale by (its ppf at (1 -+ (1 - alpha)) / 2),
+    evaluated once per distinct count; NaN where df <= 0."""
+    import scipy.stats
+    conf = 1 - alpha
+    if not 0 <= conf <= 1:
+        raise ValueError('alpha must be between 0 and 1 inclusive')
+    dof = np.asarray(runs, dtype=np.float64).reshape(-1) - 1.0
+    uniq, inv = np.unique(dof, return_inverse=True)
+    q = np.full((2, uniq.size), np.nan)
+    good = uniq > 0
+    if good.any():
+        q[:, good] = scipy.stats.t.ppf(
+            np.array([(1.0 - conf) / 2, (1.0 + conf) / 2])[:, None], uniq[good])
+    return q[0][inv], q[1][inv]
+
+
+def finalize_columns(runs, mean, m2, div, alpha):
+    """Finalized canonical averages on the device (pc_finalize_canonical):
+    runs (P,) counts, mean and m2 (P, C) float64 device tensors, div (C,)
+    divisors; returns a (4, P, C) float64 tensor of mean, std, lower and
+    upper bound."""
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    P, C = mean.shape
+    dev = mean.device
+    q_lo, q_hi = t_interval_quantiles(runs, alpha)
+    d_runs = torch.from_numpy(np.asarray(runs, dtype=np.uint32).reshape(-1)
+                              .view(np.int32)).to(dev)
+    d_q = torch.from_numpy(np.stack([q_lo, q_hi])).to(dev)
+    d_div = torch.from_numpy(np.asarray(div, dtype=np.float64)).to(dev)
+    mean = mean.contiguous()
+    m2 = m2.contiguous()
+    out = torch.empty((4, P, C), dtype=torch.float64, device=dev)
+    nat.check(lib.pc_finalize_canonical(
+        P, C, nat.ptr(d_runs), nat.ptr(mean), nat.ptr(m2), nat.ptr(d_div),
+        nat.ptr(d_q[0]), nat.ptr(d_q[1]), nat.ptr(out),
+        nat.stream_handle(torch)), 'pc_finalize_canonical')
+    return out
+
+
+def merge_moments(n_a, mean_a, m2_a, n_b, mean_b, m2_b, out=None):
+    """Chan merge of two (n, mean, M2) sets on the device
+    (pc_merge_moments): n_a, n_b (P,) float64 device tensors, the others
+    (P, C); returns (mean, m2), written into out = (mean, m2) if given (which
+    may be the a operands)."""
+    torch = nat.torch_cuda()
+    lib = nat.load_library()
+    P, C = mean_a.shape
+    if out is None:
+        out = (torch.empty_like(mean_a), torch.empty_like(m2_a))
+    nat.check(lib.pc_merge_moments(
+        P, C, nat.ptr(n_a), nat.ptr(mean_a), nat.ptr(m2_a), nat.ptr(n_b),
+        nat.ptr(mean_b), nat.ptr(m2_b), nat.ptr(out[0]), nat.ptr(out[1]),
+        nat.stream_handle(torch)), 'pc_merge_moments')
+    return out
+
+
 def micro_moments(sum1, sum2, num_edges, num_runs):
     """Mean and sample std (ddof=1) per column and step: (7, S) float64."""
     torch = nat.torch_cuda()

@@ -85,7 +85,29 @@ SIGNATURES = {
     'pc_run_moments': (ctypes.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp]),
     'pc_microcanonical_moments': (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp,
                                                  _vp, _vp]),
+    'pc_finalize_canonical': (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp,
+                                             _vp, _vp, _vp, _vp]),
+    'pc_merge_moments': (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _vp, _vp]),
+    'pc_sweep_multi_workspace_size': (ctypes.c_int, [_u32, _vp,
+                                                     ctypes.POINTER(_u64),
+                                                     ctypes.POINTER(_u64)]),
+    'pc_sweep_multi_prepare': (ctypes.c_int, [_u32, _vp, _vp, _u64]),
+    'pc_sweep_multi': (ctypes.c_int, [_u32, _vp, _vp, _u64, _vp, _u64, _vp]),
 }
+
+class SweepGraph(ctypes.Structure):
+    """pc_sweep_graph: one graph of a persistent multi-graph sweep."""
+    _fields_ = [('num_nodes', ctypes.c_uint32), ('num_edges', ctypes.c_uint32),
+                ('num_runs', ctypes.c_uint32),
+                ('span_on_first_merge', ctypes.c_int32),
+                ('merge_rounds', ctypes.c_uint32), ('reserved', ctypes.c_uint32),
+                ('d_edges', ctypes.c_void_p), ('d_node_sides', ctypes.c_void_p),
+                ('d_perm', ctypes.c_void_p), ('perm_stride', ctypes.c_int64),
+                ('d_records', ctypes.c_void_p), ('rec_stride', ctypes.c_int64),
+                ('d_span_step', ctypes.c_void_p),
+                ('d_chunk_state', ctypes.c_void_p)]
+
 
 # precisions of pc_canonical_contract
 PC_PREC_FP64 = 0

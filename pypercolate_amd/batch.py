@@ -13,16 +13,25 @@ the GPU overlaps the long large-L sweeps with the many short small-L ones.
 (torch.cuda.CUDAGraph); ``run()`` then replays it with a single launch.  All buffers are allocated up front, so a
 replay allocates nothing.
 
+With ``persistent=True`` the sweeps of all sizes are one launch instead
+(pc_sweep_multi): the permutations of every L first, then one persistent
+kernel whose resident workgroups take the runs of all lattices, largest
+first, from a device work queue, then the per-L reductions on the three
+streams.
+
 This is the batched form of the reference's jug study
 (percolate/share/jugfile.py:166-259: for L in SYSTEM_DIMENSIONS, runs ->
 reduce -> canonical averages).
 """
 
+import ctypes
+
 import numpy as np
 
 from . import _graph
 from . import _native as nat
-from ._engine import BondEngine, FusedPlan, _pipeline_streams, micro_moments
+from ._engine import (BondEngine, FusedPlan, SweepResult, _pipeline_streams,
+                      micro_moments)
 from .autotune import tune_sweep
 
 
@@ -61,17 +70,33 @@ class _LengthPlan(object):
         self.mean = None
         self.std = None
 
+    def permute(self):
+        self.engine.device_permutations(self.seed, self.runs, out=self.perm,
+                                        workspace=self.perm_ws)
+
     def launch(self):
         """Enqueue this L's pipeline on the current stream."""
         eng = self.engine
-        eng.device_permutations(self.seed, self.runs, out=self.perm,
-                                workspace=self.perm_ws)
+        self.permute()
         fused = eng.fuse_chunk_states()
         res = eng.sweep(self.perm, records=self.records, span_step=self.span,
                         workspace=self.workspace,
                         chunk_state=self.state if fused else None, chunks=fused)
         if not fused:
             res.chunk_state = eng.chunk_scan(res, out=self.state)
+        self.reduce(res)
+
+    def swept(self):
+        """The sweep result of this L's buffers (after a persistent sweep,
+        which writes records, span steps and chunk states)."""
+        res = SweepResult(self.engine, self.runs, self.records, self.span,
+                          self.perm)
+        res.chunk_state = self.state
+        return res
+
+    def reduce(self, res):
+        """Exact sums, mean per step and canonical averages of a swept L."""
+        eng = self.engine
         eng.microcanonical_sums(res, sum1=self.sum1, sum2=self.sum2)
         self.mean, self.std = micro_moments(self.sum1, self.sum2, eng.M,
                                             self.runs)
@@ -79,15 +104,29 @@ class _LengthPlan(object):
             self.x.copy_(self.mean.t())
             self.plan.run()
 
+    def sweep_graph(self):
+        """This L as a pc_sweep_graph of a persistent multi-graph sweep."""
+        eng = self.engine
+        stride = max(eng.M, 1)
+        return nat.SweepGraph(
+            num_nodes=eng.N, num_edges=eng.M, num_runs=self.runs,
+            span_on_first_merge=eng.span_first, merge_rounds=eng.merge_rounds,
+            reserved=0, d_edges=nat.ptr(eng.d_edges),
+            d_node_sides=nat.ptr(eng.d_sides), d_perm=nat.ptr(self.perm),
+            perm_stride=stride, d_records=nat.ptr(self.records),
+            rec_stride=stride, d_span_step=nat.ptr(self.span),
+            d_chunk_state=nat.ptr(self.state))
+
 
 class FiniteSizeBatch(object):
     """See module docstring."""
 
-    def __init__(self, lengths, runs, seed=42, ps=(), tune=True):
+    def __init__(self, lengths, runs, seed=42, ps=(), tune=True,
+                 persistent=False):
         self.torch = torch = nat.torch_cuda()
         if np.isscalar(runs):
             runs = [int(runs)] * len(lengths)
-        self.plans = [_LengthPlan(int(L), int(r), seed, ps, tune)
+        self.plans = [_LengthPlan(int(L), int(r), seed, ps, tune and not persistent)
                       for L, r in zip(lengths, runs)]
         # largest lattices first: their sweeps are the critical path.  Three
         # streams besides the caller's (one per hardware queue of the
@@ -100,8 +139,60 @@ class FiniteSizeBatch(object):
         self.stream_of = [min(i, len(self.streams) - 1)
                           for i in range(len(self.plans))]
         self.graph = None
+        self.persistent = bool(persistent)
+        if self.persistent:
+            self._prepare_multi()
+
+    def _prepare_multi(self):
+        """Graph table (largest L first), queue head and HBM-state slots of
+        the persistent sweep."""
+        torch = self.torch
+        lib = self.plans[0].engine.lib
+        dev = self.plans[0].engine.device
+        n = len(self.plans)
+        self.multi_graphs = (nat.SweepGraph * n)(
+            *[p.sweep_graph() for p in self.plans])
+        tb, wb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        nat.check(lib.pc_sweep_multi_workspace_size(
+            n, self.multi_graphs, ctypes.byref(tb), ctypes.byref(wb)),
+            'pc_sweep_multi_workspace_size')
+        self.multi_table = torch.empty((tb.value,), dtype=torch.uint8, device=dev)
+        self.multi_ws = (torch.empty((wb.value,), dtype=torch.uint8, device=dev)
+                         if wb.value else None)
+        nat.check(lib.pc_sweep_multi_prepare(
+            n, self.multi_graphs, nat.ptr(self.multi_table), tb.value),
+            'pc_sweep_multi_prepare')
+
+    def _launch_persistent(self):
+        torch = self.torch
+        lib = self.plans[0].engine.lib
+        main = torch.cuda.current_stream()
+        for plan in self.plans:
+            plan.permute()
+        nat.check(lib.pc_sweep_multi(
+            len(self.plans), self.multi_graphs, nat.ptr(self.multi_table),
+            int(self.multi_table.numel()), nat.ptr(self.multi_ws),
+            0 if self.multi_ws is None else int(self.multi_ws.numel()),
+            nat.stream_handle(torch)), 'pc_sweep_multi')
+        fork = torch.cuda.Event()
+        fork.record(main)
+        joins = []
+        for s in self.streams:
+            s.wait_event(fork)
+        for i, plan in enumerate(self.plans):
+            stream = self.streams[self.stream_of[i]]
+            with torch.cuda.stream(stream):
+                plan.reduce(plan.swept())
+                done = torch.cuda.Event()
+                done.record(stream)
+            joins.append(done)
+        for done in joins:
+            main.wait_event(done)
 
     def _launch_all(self):
+        if self.persistent:
+            self._launch_persistent()
+            return
         torch = self.torch
         main = torch.cuda.current_stream()
         fork = torch.cuda.Event()

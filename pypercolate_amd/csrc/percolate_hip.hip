@@ -29,8 +29,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "percolate_hip.h"
 #include "stirlerr_table.h"
@@ -1528,6 +1530,79 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
     }
 }
 
+// Persistent sweep over several graphs (the lattices of a finite-size
+// study) in one launch: a graph table, every graph's runs numbered in table
+// order (graph g owns queue items first .. first + runs - 1), and a queue
+// head in device memory.  Each workgroup takes the next item with one
+// atomic, sweeps that run with the graph's state kind (LDS codes, or 32-bit
+// codes in its own HBM slot), and repeats until the queue is empty, so long
+// and short runs pack onto the CUs with no per-graph launch or stream.  The
+// table lists the largest graphs first (their runs are the longest).  Every
+// sweep folds the chunk states in the finder wave.
+struct MultiGraph {
+    const int2 *edges;
+    const uint8_t *sides;
+    const int32_t *perm;
+    void *records;
+    uint32_t *span_step;
+    uint64_t *chunk_state;
+    int64_t perm_stride, rec_stride;
+    uint32_t N, M, runs, first, kind, span_first, rounds, shift;
+};
+#define MULTI_LDS 0u       // LDS codes, 32-bit records
+#define MULTI_HBM32 1u     // HBM codes, 32-bit records
+#define MULTI_HBM64 2u     // HBM codes, 64-bit records
+
+__global__ void zero_u32_kernel(uint32_t *p) { *p = 0u; }
+
+__global__ __launch_bounds__(128) void sweep_multi_kernel(
+    const MultiGraph *__restrict__ graphs, uint32_t num_graphs, uint32_t total,
+    uint32_t *__restrict__ head, uint32_t *__restrict__ workspace,
+    uint64_t slot_words)
+{
+    extern __shared__ uint32_t lds_words[];
+    __shared__ uint32_t item;
+    uint32_t *claims = lds_words, *handoff = lds_words + CLAIM_SLOTS;
+    uint16_t *codes = (uint16_t *)(lds_words + sweep_lds_words<true>());
+    for (;;) {
+        if (threadIdx.x == 0u)
+            item = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t q = item;
+        __syncthreads();
+        if (q >= total)
+            return;
+        uint32_t g = 0;
+        while (g + 1u < num_graphs && q >= graphs[g + 1u].first)
+            ++g;
+        const MultiGraph &G = graphs[g];
+        const uint32_t r = q - G.first;
+        const size_t cstride = (size_t)((G.M + 63u) / 64u) * PC_STATE_WORDS;
+        uint64_t *cs = G.chunk_state + r * cstride;
+        const int32_t *perm = G.perm + (int64_t)r * G.perm_stride;
+        if (G.kind == MULTI_LDS) {
+            const LdsState st{codes, 1u << G.shift, G.shift};
+            sweep_run<LdsState, uint32_t, true, true>(
+                st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
+                claims, handoff, perm, (uint32_t *)G.records + (int64_t)r * G.rec_stride,
+                G.span_step + r, cs);
+        } else {
+            const GlobalState st{workspace + (size_t)blockIdx.x * slot_words};
+            if (G.kind == MULTI_HBM32)
+                sweep_run<GlobalState, uint32_t, true, true>(
+                    st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
+                    claims, handoff, perm,
+                    (uint32_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
+            else
+                sweep_run<GlobalState, uint64_t, true, true>(
+                    st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
+                    claims, handoff, perm,
+                    (uint64_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
+        }
+        __syncthreads();
+    }
+}
+
 // Running per-run state (after some number of bonds).
 template <bool WIDE>
 struct RunState {
@@ -3008,6 +3083,64 @@ __global__ __launch_bounds__(256) void micro_ci_kernel(
     out[5 * S + s] = beta[2 * k + 1];
 }
 
+// Finalized canonical averages (percolate/hpc.py:752-834) of P rows of C
+// statistics: element i = p * C + c.  Mean and sample standard deviation
+// sqrt(M2 / (n - 1)), each divided by div[c] (the number of nodes for the
+// per-node statistics, 1 otherwise: x / 1 == x), and the Student-t interval
+// q * (std / sqrt(n)) + mean with the caller's per-row quantiles, NaN where
+// scipy's ppf refuses its arguments (n - 1 <= 0, scale not > 0, mean NaN).
+// out: [4][P][C] = mean, std, lower, upper.
+__global__ __launch_bounds__(256) void finalize_kernel(
+    uint32_t P, uint32_t C, const uint32_t *__restrict__ runs,
+    const double *__restrict__ mean, const double *__restrict__ m2,
+    const double *__restrict__ div, const double *__restrict__ q_lo,
+    const double *__restrict__ q_hi, double *__restrict__ out)
+{
+    // every product and sum rounded on its own, as numpy and scipy do
+#pragma clang fp contract(off)
+    const uint64_t PC = (uint64_t)P * C;
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= PC)
+        return;
+    const uint32_t p = (uint32_t)(i / C), c = (uint32_t)(i % C);
+    const double n = (double)runs[p];
+    const double dof = n - 1.0;
+    const double d = div[c];
+    const double mu = mean[i] / d;
+    const double sd = sqrt(m2[i] / dof) / d;
+    const double scale = sd / sqrt(n);
+    const bool ok = dof > 0.0 && scale > 0.0 && mu == mu;
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    out[i] = mu;
+    out[PC + i] = sd;
+    out[2 * PC + i] = ok ? q_lo[p] * scale + mu : nan;
+    out[3 * PC + i] = ok ? q_hi[p] * scale + mu : nan;
+}
+
+// Pairwise (n, mean, M2) merge of Chan, Golub & LeVeque (the reduction of
+// percolate/hpc.py:638-702) of P rows of C statistics: element i = p * C + c,
+// per-row counts n_a[p], n_b[p].  mean = mean_a + delta * (n_b / n),
+// M2 = M2_a + M2_b + delta^2 * (n_a n_b / n), delta = mean_b - mean_a; the
+// outputs may alias the a operands.
+__global__ __launch_bounds__(256) void merge_moments_kernel(
+    uint64_t PC, uint32_t C, const double *__restrict__ n_a,
+    const double *mean_a, const double *m2_a, const double *__restrict__ n_b,
+    const double *__restrict__ mean_b, const double *__restrict__ m2_b,
+    double *mean_out, double *m2_out)
+{
+#pragma clang fp contract(off)
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= PC)
+        return;
+    const uint64_t p = i / C;
+    const double na = n_a[p], nb = n_b[p];
+    const double n = na + nb;
+    const double ma = mean_a[i], delta = mean_b[i] - ma;
+    const double m2 = m2_a[i] + m2_b[i] + delta * delta * (na * nb / n);
+    mean_out[i] = ma + delta * (nb / n);
+    m2_out[i] = m2;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -3763,6 +3896,48 @@ extern "C" int pc_microcanonical_ci(uint32_t num_edges, uint32_t num_runs,
     return PC_OK;
 }
 
+extern "C" int pc_finalize_canonical(uint32_t num_ps, uint32_t num_columns,
+                                     const uint32_t *d_runs, const double *d_mean,
+                                     const double *d_m2, const double *d_div,
+                                     const double *d_q_lo, const double *d_q_hi,
+                                     double *d_out, void *stream)
+{
+    g_last_error[0] = 0;
+    const uint64_t total = (uint64_t)num_ps * num_columns;
+    if (total == 0)
+        return PC_OK;
+    if (!d_runs || !d_mean || !d_m2 || !d_div || !d_q_lo || !d_q_hi || !d_out)
+        return pc_fail(PC_EINVAL, "pc_finalize_canonical: NULL buffer");
+    if ((total + 255u) / 256u > 0x7FFFFFFFull)
+        return pc_fail(PC_EOVERFLOW, "pc_finalize_canonical: too many rows");
+    finalize_kernel<<<(uint32_t)((total + 255u) / 256u), 256, 0, as_stream(stream)>>>(
+        num_ps, num_columns, d_runs, d_mean, d_m2, d_div, d_q_lo, d_q_hi, d_out);
+    PC_LAUNCH_CHECK("finalize_kernel");
+    return PC_OK;
+}
+
+extern "C" int pc_merge_moments(uint32_t num_ps, uint32_t num_columns,
+                                const double *d_n_a, const double *d_mean_a,
+                                const double *d_m2_a, const double *d_n_b,
+                                const double *d_mean_b, const double *d_m2_b,
+                                double *d_mean_out, double *d_m2_out, void *stream)
+{
+    g_last_error[0] = 0;
+    const uint64_t total = (uint64_t)num_ps * num_columns;
+    if (total == 0)
+        return PC_OK;
+    if (!d_n_a || !d_mean_a || !d_m2_a || !d_n_b || !d_mean_b || !d_m2_b ||
+        !d_mean_out || !d_m2_out)
+        return pc_fail(PC_EINVAL, "pc_merge_moments: NULL buffer");
+    if ((total + 255u) / 256u > 0x7FFFFFFFull)
+        return pc_fail(PC_EOVERFLOW, "pc_merge_moments: too many rows");
+    merge_moments_kernel<<<(uint32_t)((total + 255u) / 256u), 256, 0, as_stream(stream)>>>(
+        total, num_columns, d_n_a, d_mean_a, d_m2_a, d_n_b, d_mean_b, d_m2_b,
+        d_mean_out, d_m2_out);
+    PC_LAUNCH_CHECK("merge_moments_kernel");
+    return PC_OK;
+}
+
 extern "C" int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
                                          const uint64_t *d_sum1,
                                          const uint64_t *d_sum2, double *d_mean,
@@ -3778,6 +3953,156 @@ extern "C" int pc_microcanonical_moments(uint32_t num_edges, uint32_t num_runs,
                            as_stream(stream)>>>(total, num_runs, d_sum1, d_sum2,
                                                 d_mean, d_std);
     PC_LAUNCH_CHECK("micro_moments_kernel");
+    return PC_OK;
+}
+
+// Plan of a persistent multi-graph sweep: device table entries, the LDS
+// bytes (the largest LDS-state graph), the HBM slot size (the largest
+// HBM-state graph), the queue length and the grid (workgroups resident at
+// once).
+struct MultiPlan {
+    std::vector<MultiGraph> table;
+    size_t lds;
+    uint64_t slot_words, total;
+    uint32_t grid;
+};
+
+static int multi_plan(uint32_t num_graphs, const pc_sweep_graph *graphs,
+                      MultiPlan &plan)
+{
+    if (num_graphs == 0 || !graphs)
+        return pc_fail(PC_EINVAL, "pc_sweep_multi: no graphs");
+    plan.table.assign(num_graphs, MultiGraph{});
+    plan.lds = sweep_lds_words<true>() * sizeof(uint32_t);
+    plan.slot_words = 0;
+    plan.total = 0;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+        const pc_sweep_graph &in = graphs[g];
+        MultiGraph &G = plan.table[g];
+        const uint32_t rounds = in.merge_rounds ? in.merge_rounds : g_merge_rounds;
+        if (in.num_nodes == 0 || in.num_nodes > kSweepMaxNodes || rounds > 64)
+            return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: bad sizes", g);
+        if (in.num_runs && in.num_edges &&
+            (!in.d_edges || !in.d_perm || !in.d_records || !in.d_span_step ||
+             !in.d_chunk_state || in.perm_stride < (int64_t)in.num_edges ||
+             in.rec_stride < (int64_t)in.num_edges))
+            return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: bad buffers", g);
+        if (in.num_runs && !in.d_span_step)
+            return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: no span output", g);
+        G.edges = (const int2 *)in.d_edges;
+        G.sides = in.d_node_sides;
+        G.perm = in.d_perm;
+        G.records = in.d_records;
+        G.span_step = in.d_span_step;
+        G.chunk_state = in.d_chunk_state;
+        G.perm_stride = in.perm_stride;
+        G.rec_stride = in.rec_stride;
+        G.N = in.num_nodes;
+        G.M = in.num_edges;
+        G.runs = in.num_runs;
+        G.first = (uint32_t)plan.total;
+        G.span_first = in.span_on_first_merge ? 1u : 0u;
+        G.rounds = rounds;
+        G.shift = in.num_nodes > 1u ? 32u - (uint32_t)__builtin_clz(in.num_nodes - 1u) : 0u;
+        if (in.num_nodes <= kLdsMaxNodes) {
+            G.kind = MULTI_LDS;
+            plan.lds = std::max(plan.lds, sweep_lds_bytes<true>(in.num_nodes));
+        } else {
+            G.kind = in.num_nodes <= kU32RecordMaxNodes ? MULTI_HBM32 : MULTI_HBM64;
+            plan.slot_words = std::max<uint64_t>(plan.slot_words, in.num_nodes);
+        }
+        plan.total += in.num_runs;
+    }
+    if (plan.total >= (1ull << 32) - 65536u)
+        return pc_fail(PC_EOVERFLOW, "pc_sweep_multi: too many runs");
+    int dev = 0, cus = 0, blocks = 0;
+    PC_HIP_CHECK(hipGetDevice(&dev));
+    PC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, sweep_multi_kernel, 128, plan.lds));
+    if (blocks < 1)
+        return pc_fail(PC_EINVAL, "pc_sweep_multi: %zu bytes of LDS do not fit", plan.lds);
+    uint64_t grid = (uint64_t)cus * (uint64_t)blocks;
+    if (grid > plan.total)
+        grid = plan.total ? plan.total : 1u;
+    plan.grid = (uint32_t)grid;
+    return PC_OK;
+}
+
+// The table, then the queue head at the next 256-byte boundary.
+static uint64_t multi_head_offset(uint32_t num_graphs)
+{
+    return ((uint64_t)num_graphs * sizeof(MultiGraph) + 255u) & ~(uint64_t)255u;
+}
+static uint64_t multi_table_bytes(uint32_t num_graphs)
+{
+    return multi_head_offset(num_graphs) + 256u;
+}
+
+extern "C" int pc_sweep_multi_workspace_size(uint32_t num_graphs,
+                                             const pc_sweep_graph *graphs,
+                                             uint64_t *table_bytes,
+                                             uint64_t *workspace_bytes)
+{
+    g_last_error[0] = 0;
+    if (!table_bytes || !workspace_bytes)
+        return pc_fail(PC_EINVAL, "pc_sweep_multi_workspace_size: NULL output");
+    MultiPlan plan;
+    const int rc = multi_plan(num_graphs, graphs, plan);
+    if (rc)
+        return rc;
+    *table_bytes = multi_table_bytes(num_graphs);
+    *workspace_bytes = (uint64_t)plan.grid * plan.slot_words * 4u;
+    return PC_OK;
+}
+
+extern "C" int pc_sweep_multi_prepare(uint32_t num_graphs,
+                                      const pc_sweep_graph *graphs,
+                                      void *d_table, uint64_t table_bytes)
+{
+    g_last_error[0] = 0;
+    MultiPlan plan;
+    const int rc = multi_plan(num_graphs, graphs, plan);
+    if (rc)
+        return rc;
+    const size_t bytes = (size_t)num_graphs * sizeof(MultiGraph);
+    if (!d_table || table_bytes < multi_table_bytes(num_graphs) ||
+        ((uintptr_t)d_table & 255u))
+        return pc_fail(PC_ENOMEM, "pc_sweep_multi_prepare: table needs %llu bytes, "
+                       "256-byte aligned",
+                       (unsigned long long)multi_table_bytes(num_graphs));
+    PC_HIP_CHECK(hipMemcpy(d_table, plan.table.data(), bytes, hipMemcpyHostToDevice));
+    return PC_OK;
+}
+
+extern "C" int pc_sweep_multi(uint32_t num_graphs, const pc_sweep_graph *graphs,
+                              void *d_table, uint64_t table_bytes,
+                              void *d_workspace, uint64_t workspace_bytes,
+                              void *stream)
+{
+    g_last_error[0] = 0;
+    MultiPlan plan;
+    const int rc = multi_plan(num_graphs, graphs, plan);
+    if (rc)
+        return rc;
+    if (!d_table || table_bytes < multi_table_bytes(num_graphs) ||
+        ((uintptr_t)d_table & 255u))
+        return pc_fail(PC_EINVAL, "pc_sweep_multi: table of %llu bytes too small "
+                       "or not 256-byte aligned", (unsigned long long)table_bytes);
+    const uint64_t need = (uint64_t)plan.grid * plan.slot_words * 4u;
+    if (need && (!d_workspace || workspace_bytes < need))
+        return pc_fail(PC_ENOMEM, "pc_sweep_multi: workspace needs %llu bytes",
+                       (unsigned long long)need);
+    if (plan.total == 0)
+        return PC_OK;
+    uint32_t *head = (uint32_t *)((char *)d_table + multi_head_offset(num_graphs));
+    hipStream_t s = as_stream(stream);
+    zero_u32_kernel<<<1, 1, 0, s>>>(head);
+    PC_LAUNCH_CHECK("zero_u32_kernel");
+    sweep_multi_kernel<<<plan.grid, 128, plan.lds, s>>>(
+        (const MultiGraph *)d_table, num_graphs, (uint32_t)plan.total, head,
+        (uint32_t *)d_workspace, plan.slot_words);
+    PC_LAUNCH_CHECK("sweep_multi_kernel");
     return PC_OK;
 }
 

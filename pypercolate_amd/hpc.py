@@ -12,7 +12,6 @@ numpy RandomState permutation.  Extensions for batches of runs carry a
 """
 
 import numpy as np
-import scipy.stats
 
 from . import _graph
 
@@ -343,7 +342,10 @@ def finalize_canonical_averages(
     mean, the sample standard deviation sqrt(M2 / (n - 1)) and the Student-t
     confidence interval of the mean; max_cluster_size becomes the
     percolation strength, and it and the moments are divided by the number
-    of nodes."""
+    of nodes.  Computed on the device (pc_finalize_canonical); scipy supplies
+    the t quantiles of the distinct run counts."""
+    from . import _engine as E
+    torch = E.nat.torch_cuda()
     names = canonical_averages.dtype.names
     present = _statistics_in(names, ('_mean', '_m2'))
     spanning = any(st[0] == 'percolation_probability' for st in present)
@@ -355,18 +357,30 @@ def finalize_canonical_averages(
     ret['number_of_runs'] = runs
     ret['p'] = ps
     ret['alpha'] = alpha
-    for field, final, per_node in present:
-        mean = np.array(canonical_averages[field + '_mean'], dtype=np.float64)
-        n = _per_row(runs, mean)
-        std = np.sqrt(canonical_averages[field + '_m2'] / (n - 1))
-        if per_node:
-            mean = mean / number_of_nodes
-            std = std / number_of_nodes
-        lower, upper = scipy.stats.t.interval(
-            1 - alpha, df=n - 1, loc=mean, scale=std / np.sqrt(n))
-        ret[final + '_mean'] = mean
-        ret[final + '_std'] = std
-        ret[final + '_ci'] = np.stack([lower, upper], axis=-1)
+    rows = int(np.prod(ret.shape))
+    flat = canonical_averages.reshape(rows)
+    widths = [int(np.prod(flat.dtype[field + '_mean'].shape) or 1)
+              for field, _, _ in present]
+    mean = np.concatenate([np.asarray(flat[f + '_mean'], np.float64).reshape(rows, -1)
+                           for f, _, _ in present], axis=1)
+    m2 = np.concatenate([np.asarray(flat[f + '_m2'], np.float64).reshape(rows, -1)
+                         for f, _, _ in present], axis=1)
+    div = np.concatenate([np.full(w, float(number_of_nodes) if per_node else 1.0)
+                          for (_, _, per_node), w in zip(present, widths)])
+    dev = torch.device('cuda', torch.cuda.current_device())
+    out = E.finalize_columns(np.asarray(runs).reshape(rows),
+                             torch.from_numpy(mean).to(dev),
+                             torch.from_numpy(m2).to(dev), div,
+                             alpha).cpu().numpy()
+    off = 0
+    for (_, final, _), w in zip(present, widths):
+        shape = ret.shape + ret.dtype[final + '_mean'].shape
+        sl = slice(off, off + w)
+        ret[final + '_mean'] = out[0][:, sl].reshape(shape)
+        ret[final + '_std'] = out[1][:, sl].reshape(shape)
+        ret[final + '_ci'] = np.stack([out[2][:, sl], out[3][:, sl]],
+                                      axis=-1).reshape(shape + (2,))
+        off += w
     return ret
 
 
@@ -460,14 +474,20 @@ def bond_canonical_averages_batch(
         nat.check(lib.pc_run_moments(nat.ptr(q), ps.size, rc, cols,
                                      nat.ptr(mean), nat.ptr(m2),
                                      nat.stream_handle(torch)), 'pc_run_moments')
-        mean = mean.cpu().numpy()
-        m2 = m2.cpu().numpy()
         if n_acc is None:
             n_acc, mean_acc, m2_acc = float(rc), mean, m2
         else:
-            n_acc, mean_acc, m2_acc = _online_variance(
-                (n_acc, mean_acc, m2_acc), (float(rc), mean, m2))
+            # Chan merge on the device, into the accumulators
+            n_a = torch.full((ps.size,), n_acc, dtype=torch.float64,
+                             device=eng.device)
+            n_b = torch.full((ps.size,), float(rc), dtype=torch.float64,
+                             device=eng.device)
+            E.merge_moments(n_a, mean_acc, m2_acc, n_b, mean, m2,
+                            out=(mean_acc, m2_acc))
+            n_acc += float(rc)
         del x, q, res, d_perm
+    mean_acc = mean_acc.cpu().numpy()
+    m2_acc = m2_acc.cpu().numpy()
 
     ret = np.empty(ps.size, dtype=canonical_averages_dtype(spanning_cluster))
     ret['number_of_runs'] = total

@@ -48,6 +48,58 @@ def test_batch_graph_matches_unbatched(gpu, tmp_path, monkeypatch):
                 assert np.array_equal(got[L][key], eager[L][key]), (k, L, key)
 
 
+def test_persistent_batch_matches_per_length(gpu):
+    """One persistent sweep over lattices of every state kind (LDS codes with
+    32-bit records at L = 8, 16; HBM codes with 32-bit records at L = 130,
+    64-bit at L = 257) gives the per-L path's exact sums, means and
+    canonical averages bit for bit, eagerly and on every HIP-graph replay."""
+    lengths, runs = [8, 16, 130, 257], [96, 40, 5, 3]
+    plain = FiniteSizeBatch(lengths, runs, seed=7, ps=PS, tune=False)
+    plain.run_sequential()
+    gpu.cuda.synchronize()
+    want = plain.results()
+    pers = FiniteSizeBatch(lengths, runs, seed=7, ps=PS, persistent=True)
+    assert [p.length for p in pers.plans] == [257, 130, 16, 8]
+    pers.run()
+    gpu.cuda.synchronize()
+    for L in lengths:
+        got = pers.results()[L]
+        for key in ('sum1', 'sum2', 'mean', 'canonical'):
+            assert np.array_equal(got[key], want[L][key]), (L, key)
+        plan = [p for p in pers.plans if p.length == L][0]
+        ref = [p for p in plain.plans if p.length == L][0]
+        assert np.array_equal(plan.records.cpu().numpy(), ref.records.cpu().numpy())
+        assert np.array_equal(plan.span.cpu().numpy(), ref.span.cpu().numpy())
+    pers.capture()
+    for k in range(3):
+        pers.run()
+        gpu.cuda.synchronize()
+        junk = gpu.ones(1 << 20, device='cuda').sum().item()
+        assert junk == float(1 << 20)
+        got = pers.results()
+        for L in lengths:
+            for key in ('sum1', 'sum2', 'mean', 'canonical'):
+                assert np.array_equal(got[L][key], want[L][key]), (k, L, key)
+
+
+def test_persistent_sweep_rejects_bad_tables(gpu):
+    """Host-side checks of pc_sweep_multi: no graphs, a table that is too
+    small, a missing output buffer."""
+    import ctypes
+    lib = nat.load_library()
+    tb, wb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    assert lib.pc_sweep_multi_workspace_size(0, None, ctypes.byref(tb),
+                                             ctypes.byref(wb)) == -1
+    batch = FiniteSizeBatch([8], [4], seed=1, persistent=True)
+    g = batch.multi_graphs
+    t = batch.multi_table
+    assert lib.pc_sweep_multi(1, g, nat.ptr(t), 16, None, 0, None) == -1
+    bad = (nat.SweepGraph * 1)(g[0])
+    bad[0].d_span_step = None
+    assert lib.pc_sweep_multi_workspace_size(1, bad, ctypes.byref(tb),
+                                             ctypes.byref(wb)) == -1
+
+
 @pytest.mark.parametrize('L', [16, 64, 130])
 def test_tuned_knobs_do_not_change_records(gpu, tmp_path, monkeypatch, L):
     monkeypatch.setenv('PYPERCOLATE_AMD_CACHE', str(tmp_path))

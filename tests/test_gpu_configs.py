@@ -5,7 +5,8 @@ of the benchmark configurations.
   L (262144 ... 256), Philox permutations; the HIP-graph replays equal the
   eager per-L pipelines bit for bit, and every L's exact per-step sums over
   all its runs equal the oracle's over the same (device-drawn)
-  permutations.  The reference runs this study one size at a time
+  permutations; the persistent form (one sweep launch over all sizes)
+  gives the same results.  The reference runs this study one size at a time
   (percolate/share/jugfile.py:166-259).
 * Canonical averages at L = 1024: M = 2,095,104, 4096 p in [0.3, 0.7], the
   14 columns of the benchmark (mean and mean square of the seven
@@ -45,6 +46,16 @@ def test_finite_size_batch_at_size(gpu):
         for L in LENGTHS:
             for key in ('sum1', 'sum2', 'mean', 'canonical'):
                 assert np.array_equal(got[L][key], eager[L][key]), (k, L, key)
+    # one persistent sweep launch over all six lattices: the same sums,
+    # means and canonical averages
+    pers = FiniteSizeBatch(LENGTHS, runs, seed=42, ps=ps, persistent=True)
+    pers.run()
+    torch.cuda.synchronize()
+    got = pers.results()
+    for L in LENGTHS:
+        for key in ('sum1', 'sum2', 'mean', 'canonical'):
+            assert np.array_equal(got[L][key], eager[L][key]), ('persistent', L, key)
+    del pers, got
     for plan in batch.plans:
         L, R = plan.length, plan.runs
         g = lattice_oracle_graph(L)
