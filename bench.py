@@ -553,7 +553,9 @@ def main(argv=None):
     tuning = tune_sweep(eng) if args.tune else None
     R, M, N, S = args.runs, eng.M, eng.N, eng.S
     dev = eng.device
-    pipe = Pipeline(eng, R, seed=args.seed)
+    # each step also draws the next step's permutations once its sweeps have
+    # ended, beside its last reductions (the first step draws its own)
+    pipe = Pipeline(eng, R, seed=args.seed, prefetch=True, prefetch_after=0)
     sum1 = pipe.sum1
 
     for _ in range(args.warmup):
@@ -621,10 +623,12 @@ def main(argv=None):
                              args.length, N, M, R),
             'length': args.length, 'runs': R, 'nodes': N, 'edges': M,
             'parallelism': ('single GPU, one run per workgroup of {} '
-                            'wavefront(s); permutations of all runs first, '
-                            'then sub-batches of {} runs (one residency), '
-                            'sweeps alternating between two streams, '
-                            'reductions beside them on a third').format(
+                            'wavefront(s); permutations of all runs (one '
+                            'launch; every step draws the next step\'s once '
+                            'its sweeps have ended, beside its last '
+                            'reductions), sub-batches of {} runs (one '
+                            'residency), sweeps alternating between two '
+                            'streams, reductions beside them on a third').format(
                                 2 if N >= 4096 else 1, pipe.sub),
         },
         'roofline': {

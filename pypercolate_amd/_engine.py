@@ -303,10 +303,18 @@ class Pipeline(object):
 
     Permutations: Philox-keyed (pc_permute_philox, seed) by default, or
     with seeds= (R integers) numpy RandomState(seeds[r]).permutation(M),
-    bit-identical to the reference's draw, on the device."""
+    bit-identical to the reference's draw, on the device.
+
+    prefetch (Philox permutations drawn ahead): each run() also draws the
+    permutations of the next run() into a second buffer, on the caller's
+    stream once the sweep of sub-batch ``-1 - prefetch_after`` has ended,
+    so that they fill the CUs the last sweeps and the reductions leave free;
+    the next run() starts with its sweeps.  The first run() draws its own.
+    Every run() still draws one batch of permutations, the same ones (the
+    permutations depend only on the seed and the run index)."""
 
     def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None,
-                 permute_ahead=None):
+                 permute_ahead=None, prefetch=False, prefetch_after=1):
         torch = self.torch = engine.torch
         self.engine = eng = engine
         self.R = int(num_runs)
@@ -339,7 +347,13 @@ class Pipeline(object):
         # waits for LDS they hold) or per sub-batch beside the sweeps
         self.permute_ahead = (seeds is None) if permute_ahead is None else bool(permute_ahead)
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
-        self.perm = torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
+        self.prefetch = bool(prefetch) and self.permute_ahead and seeds is None
+        self.prefetch_after = max(0, int(prefetch_after))
+        self.perms = [torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
+                      for _ in range(2 if self.prefetch else 1)]
+        self.cur = 0
+        self.prefetched = False
+        self.perm = self.perms[0]
         self.records = torch.empty((self.R, max(M, 1)), dtype=eng.rec_torch,
                                    device=dev)
         self.span = torch.empty((self.R,), dtype=torch.int32, device=dev)
@@ -374,15 +388,19 @@ class Pipeline(object):
         self.sweep_events = []
         first = True
         fused = eng.fuse_chunk_states()
+        self.perm = self.perms[self.cur]
         if self.permute_ahead:
             with torch.cuda.stream(sp):
-                if self.d_seeds is None:
+                if self.prefetched:
+                    pass        # drawn by the previous run(), earlier on sp
+                elif self.d_seeds is None:
                     eng.device_permutations(self.seed, self.R, out=self.perm,
                                             workspace=self.perm_ws)
                 else:
                     eng.randomstate_permutations(self.d_seeds, out=self.perm)
                 have_all = torch.cuda.Event()
                 have_all.record(sp)
+        nsub = (self.R + self.sub - 1) // self.sub
         for i, b0 in enumerate(range(0, self.R, self.sub)):
             n = min(self.sub, self.R - b0)
             ss = (ss0, ss1)[i & 1]
@@ -417,6 +435,17 @@ class Pipeline(object):
                 eng.microcanonical_sums(res, sum1=self.sum1, sum2=self.sum2,
                                         accumulate=not first)
             first = False
+            if self.prefetch and i == max(0, nsub - 1 - self.prefetch_after):
+                # the next run()'s permutations, into the other buffer (the
+                # run before this one, which read it, has ended)
+                sp.wait_event(e1)
+                with torch.cuda.stream(sp):
+                    eng.device_permutations(self.seed, self.R,
+                                            out=self.perms[1 - self.cur],
+                                            workspace=self.perm_ws)
+        if self.prefetch:
+            self.cur = 1 - self.cur
+            self.prefetched = True
         for s in self.streams:
             done = torch.cuda.Event()
             done.record(s)

@@ -194,6 +194,29 @@ def test_pipeline_matches_single_launch(gpu):
         assert np.array_equal(s2.cpu().numpy(), ref[1])
 
 
+@pytest.mark.parametrize('L,after', [(32, 1), (32, 0), (32, 9), (130, 1)])
+def test_pipeline_prefetch_matches_single_launch(gpu, L, after):
+    """Permutations of the next run() drawn during this one (prefetch):
+    every run() gives the one-launch exact sums, also with other work on
+    the caller's stream between runs and for the HBM-state sweep."""
+    from pypercolate_amd._engine import Pipeline
+    eng = BondEngine(_graph.square_lattice(L))
+    R = 3000 if L == 32 else 300
+    res = eng.sweep(eng.device_permutations(9, R))
+    ref = [t.cpu().numpy() for t in eng.microcanonical_sums(res)]
+    pipe = Pipeline(eng, R, seed=9, sub_batch=R // 4 + 1, prefetch=True,
+                    prefetch_after=after)
+    assert pipe.prefetch
+    for k in range(4):
+        s1, s2 = pipe.run()
+        junk = gpu.ones(1 << 20, device='cuda').sum().item()
+        assert junk == float(1 << 20)
+        gpu.cuda.synchronize()
+        assert np.array_equal(s1.cpu().numpy(), ref[0]), k
+        assert np.array_equal(s2.cpu().numpy(), ref[1]), k
+        assert np.array_equal(pipe.perm.cpu().numpy(), res.perm.cpu().numpy()), k
+
+
 def test_pipeline_bit_exact_mode_matches_oracle(gpu):
     """The pipeline with device RandomState permutations (seeds = run
     index, the L = 16 configuration) gives the oracle's exact sums over
