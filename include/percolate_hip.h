@@ -76,6 +76,27 @@ int pc_permute_philox(uint64_t seed, uint64_t first_run, uint32_t num_runs,
                       void *stream);
 
 /*
+ * pc_sweep for the square lattice of side `length` (1..16384) without an
+ * edge array: each bond's endpoints are computed from its index, in the
+ * edge order of the closed-form lattice (rows r = 0..L-1 of nodes
+ * (r, c); in every row but the last, node (r, c)'s edge down to (r + 1, c),
+ * then right to (r, c + 1) when c < L - 1; the last row's L - 1 edges to the
+ * right) -- networkx's order for spanning_2d_grid(L)'s percolation graph,
+ * percolate/percolate.py:899-965.  Node (r, c) is r L + c, or with `tiled`
+ * (L a multiple of 8) ((r / 4)(L / 8) + c / 8) 32 + (r mod 4) 8 + c mod 8,
+ * 4 x 8 tiles of one 128-byte line each.  N = L^2, M = 2 L (L - 1); the
+ * other arguments and outputs are pc_sweep's.  Records equal pc_sweep's
+ * over the same edges.
+ */
+int pc_sweep_lattice(uint32_t length, int32_t tiled,
+                     const uint8_t *d_node_sides, int32_t span_on_first_merge,
+                     const int32_t *d_perm, int64_t perm_stride,
+                     uint32_t num_runs, void *d_records, int64_t rec_stride,
+                     uint32_t *d_span_step, void *d_workspace,
+                     uint64_t workspace_bytes, uint32_t merge_rounds,
+                     uint32_t state_mode, uint64_t *d_chunk_state, void *stream);
+
+/*
  * One graph of a persistent multi-graph sweep (pc_sweep_multi): the
  * arguments pc_sweep takes for one graph, with chunk states always written.
  */
@@ -83,7 +104,10 @@ typedef struct pc_sweep_graph {
     uint32_t num_nodes, num_edges, num_runs;
     int32_t span_on_first_merge;
     uint32_t merge_rounds;          /* 0: the library default */
-    uint32_t reserved;
+    uint32_t lattice_length;        /* 0: the edges are d_edges; L: the square
+                                       lattice of side L, edges computed as
+                                       pc_sweep_lattice does (tiled order when
+                                       num_nodes > 16384), d_edges unused */
     const int32_t *d_edges;         /* [num_edges][2] */
     const uint8_t *d_node_sides;    /* [num_nodes] spanning sides, or NULL */
     const int32_t *d_perm;          /* [num_runs][perm_stride] */

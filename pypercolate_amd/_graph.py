@@ -44,6 +44,9 @@ class GraphArrays(object):
             raise ValueError('node_sides must hold one value 0, 1 or 2 per node')
         self.nodes = nodes
         self._edge_list = edge_list
+        # side L when this is square_lattice(L)'s closed form (its edges then
+        # follow from their index, pc_sweep_lattice), else 0
+        self.lattice_length = 0
 
     @property
     def spanning(self):
@@ -218,7 +221,9 @@ def square_lattice(length, spanning_cluster=True):
             node_sides[idx[0]] = 1
             node_sides[idx[L - 1]] = 2
     nodes = [(i, j) for i in range(1, L + 1) for j in range(L)]
-    return GraphArrays(N, edges, node_sides, span_first, nodes=nodes)
+    ga = GraphArrays(N, edges, node_sides, span_first, nodes=nodes)
+    ga.lattice_length = L
+    return ga
 
 
 def linear_chain(length, spanning_cluster=True):

@@ -94,6 +94,9 @@ SIGNATURES = {
                                                      ctypes.POINTER(_u64)]),
     'pc_sweep_multi_prepare': (ctypes.c_int, [_u32, _vp, _vp, _u64]),
     'pc_sweep_multi': (ctypes.c_int, [_u32, _vp, _vp, _u64, _vp, _u64, _vp]),
+    'pc_sweep_lattice': (ctypes.c_int, [_u32, _i32, _vp, _i32, _vp, _i64, _u32,
+                                        _vp, _i64, _vp, _vp, _u64, _u32, _u32,
+                                        _vp, _vp]),
 }
 
 class SweepGraph(ctypes.Structure):
@@ -101,7 +104,7 @@ class SweepGraph(ctypes.Structure):
     _fields_ = [('num_nodes', ctypes.c_uint32), ('num_edges', ctypes.c_uint32),
                 ('num_runs', ctypes.c_uint32),
                 ('span_on_first_merge', ctypes.c_int32),
-                ('merge_rounds', ctypes.c_uint32), ('reserved', ctypes.c_uint32),
+                ('merge_rounds', ctypes.c_uint32), ('lattice_length', ctypes.c_uint32),
                 ('d_edges', ctypes.c_void_p), ('d_node_sides', ctypes.c_void_p),
                 ('d_perm', ctypes.c_void_p), ('perm_stride', ctypes.c_int64),
                 ('d_records', ctypes.c_void_p), ('rec_stride', ctypes.c_int64),

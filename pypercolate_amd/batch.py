@@ -111,7 +111,11 @@ class _LengthPlan(object):
         return nat.SweepGraph(
             num_nodes=eng.N, num_edges=eng.M, num_runs=self.runs,
             span_on_first_merge=eng.span_first, merge_rounds=eng.merge_rounds,
-            reserved=0, d_edges=nat.ptr(eng.d_edges),
+            # computed lattice edges for the HBM-state lattices (in the tiled
+            # order the persistent sweep assumes there), as the engine uses
+            lattice_length=(eng.lattice_length
+                            if eng.lattice_sweep and eng.lattice_tiled else 0),
+            d_edges=nat.ptr(eng.d_edges),
             d_node_sides=nat.ptr(eng.d_sides), d_perm=nat.ptr(self.perm),
             perm_stride=stride, d_records=nat.ptr(self.records),
             rec_stride=stride, d_span_step=nat.ptr(self.span),

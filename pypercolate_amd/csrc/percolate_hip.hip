@@ -1166,9 +1166,56 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 // still an ancestor: the resolver's find continues from it, usually with a
 // single read.  With chunk_out, the finder also folds the records of batch
 // b - 1 into the running state, so the reductions stay off the resolver.
-template <class St, typename RecT, bool CHUNKS, bool SPLIT>
+// Edge sources of the sweep: edge e -> its endpoints (u, v).
+// EdgeList reads the graph's edge array.  LatticeEdges computes them for the
+// square lattice of side L in the order square_lattice builds (rows r =
+// 0..L-1; in each row but the last, node (r, c)'s edge to (r + 1, c), then
+// to (r, c + 1) when c < L - 1; the last row's L - 1 edges to the right),
+// with node (r, c) stored at r L + c, or, tiled (L a multiple of 8), in
+// 4 x 8 tiles of one 128-byte line each: ((r / 4) (L / 8) + c / 8) 32 +
+// (r mod 4) 8 + c mod 8, the order of the host's locality relabelling.  No
+// memory is read for an edge.
+struct EdgeList {
+    const int2 *__restrict__ p;
+    __device__ __forceinline__ int2 operator()(int32_t e) const { return p[e]; }
+};
+
+struct LatticeEdges {
+    uint32_t L, d;   // side, edges per row but the last (2 L - 1)
+    float inv_d;     // 1 / d, a first guess of the row
+    uint32_t tiled;
+    __device__ __forceinline__ uint32_t node(uint32_t r, uint32_t c) const
+    {
+        return tiled ? ((((r >> 2) * (L >> 3) + (c >> 3)) << 5) | ((r & 3u) << 3) | (c & 7u))
+                     : r * L + c;
+    }
+    __device__ __forceinline__ int2 operator()(int32_t e) const
+    {
+        const uint32_t x = (uint32_t)e;
+        uint32_t r = (uint32_t)((float)x * inv_d);
+        // the float guess is off by at most one row either way
+        if (r * d > x)
+            --r;
+        else if ((r + 1u) * d <= x)
+            ++r;
+        const uint32_t q = x - r * d;
+        uint32_t r2 = r, c = q, c2 = q + 1u;   // the last row: to the right
+        if (r + 1u < L) {
+            c = q >> 1;
+            if (q & 1u)
+                c2 = c + 1u;   // to the right
+            else {
+                r2 = r + 1u;   // down
+                c2 = c;
+            }
+        }
+        return make_int2((int32_t)node(r, c), (int32_t)node(r2, c2));
+    }
+};
+
+template <class St, typename RecT, bool CHUNKS, bool SPLIT, class ES>
 __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
-                                          const int2 *__restrict__ edges,
+                                          const ES &edges,
                                           const uint8_t *__restrict__ sides,
                                           bool span_first, uint32_t rounds,
                                           uint32_t *__restrict__ claims,
@@ -1211,7 +1258,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     if (!SPLIT) {
         e_ahead = (64u + lane < M) ? perm[64u + lane] : 0;
         if (lane < M)
-            uv_next = edges[perm[lane]];
+            uv_next = edges(perm[lane]);
     }
     // codes of the next batch's endpoints, read one batch early: parent
     // links only ever move up the tree (a union links a root under a root,
@@ -1233,9 +1280,9 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     uint32_t f_hu = 0xFFFFFFFFu, f_hv = 0xFFFFFFFFu;
     if (SPLIT && !resolver) {
         if (lane < M)
-            f_uv1 = edges[perm[lane]];
+            f_uv1 = edges(perm[lane]);
         if (64u + lane < M)
-            f_uv2 = edges[perm[64u + lane]];
+            f_uv2 = edges(perm[64u + lane]);
         f_e3 = (128u + lane < M) ? perm[128u + lane] : 0;
     }
     auto find_batch = [&](uint32_t b0) {
@@ -1246,7 +1293,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
             f_hu = st.get((uint32_t)f_uv1.x);
             f_hv = st.get((uint32_t)f_uv1.y);
         }
-        f_uv2 = (b0 + 128u + lane < M) ? edges[f_e3] : make_int2(0, 0);
+        f_uv2 = (b0 + 128u + lane < M) ? edges(f_e3) : make_int2(0, 0);
         f_e3 = (b0 + 192u + lane < M) ? perm[b0 + 192u + lane] : 0;
         uint32_t x = st.is_root(hu) ? (uint32_t)uv.x : hu;
         uint32_t y = st.is_root(hv) ? (uint32_t)uv.y : hv;
@@ -1306,7 +1353,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
             find2(st, ru, cu, rv, cv);
         } else {
             const int2 uv = uv_next;
-            uv_next = (n + 64u < M) ? edges[e_ahead] : make_int2(0, 0);
+            uv_next = (n + 64u < M) ? edges(e_ahead) : make_int2(0, 0);
             e_ahead = (n + 128u < M) ? perm[n + 128u] : 0;
             ru = st.is_root(hint_u) ? (uint32_t)uv.x : hint_u;
             rv = st.is_root(hint_v) ? (uint32_t)uv.y : hint_v;
@@ -1486,9 +1533,9 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 template <bool SPLIT>
 constexpr uint32_t sweep_lds_words() { return CLAIM_SLOTS + (SPLIT ? SWEEP_HANDOFF_WORDS : 0u); }
 
-template <typename RecT, bool CHUNKS, bool SPLIT>
+template <typename RecT, bool CHUNKS, bool SPLIT, class ES>
 __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
-    uint32_t N, uint32_t M, const int2 *__restrict__ edges,
+    uint32_t N, uint32_t M, ES edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
@@ -1501,16 +1548,16 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const uint32_t shift = N > 1u ? 32u - (uint32_t)__builtin_clz(N - 1u) : 0u;
     const LdsState st{(uint16_t *)(lds_words + sweep_lds_words<SPLIT>()), 1u << shift, shift};
-    sweep_run<LdsState, RecT, CHUNKS, SPLIT>(
+    sweep_run<LdsState, RecT, CHUNKS, SPLIT, ES>(
         st, N, M, edges, sides, span_first != 0, rounds, lds_words,
         lds_words + CLAIM_SLOTS, perm + (int64_t)r * perm_stride,
         records + (int64_t)r * rec_stride, span_step + r,
         CHUNKS ? chunk_state + r * cstride : nullptr);
 }
 
-template <typename RecT, bool CHUNKS, bool SPLIT>
+template <typename RecT, bool CHUNKS, bool SPLIT, class ES>
 __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
-    uint32_t N, uint32_t M, uint32_t R, const int2 *__restrict__ edges,
+    uint32_t N, uint32_t M, uint32_t R, ES edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
     RecT *__restrict__ records, int64_t rec_stride,
@@ -1522,7 +1569,7 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
-        sweep_run<GlobalState, RecT, CHUNKS, SPLIT>(
+        sweep_run<GlobalState, RecT, CHUNKS, SPLIT, ES>(
             st, N, M, edges, sides, span_first != 0, rounds, claims, handoff,
             perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
             span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
@@ -1541,6 +1588,7 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
 // sweep folds the chunk states in the finder wave.
 struct MultiGraph {
     const int2 *edges;
+    LatticeEdges lattice;   // used when lattice.L != 0 (edges unused)
     const uint8_t *sides;
     const int32_t *perm;
     void *records;
@@ -1580,25 +1628,31 @@ __global__ __launch_bounds__(128) void sweep_multi_kernel(
         const size_t cstride = (size_t)((G.M + 63u) / 64u) * PC_STATE_WORDS;
         uint64_t *cs = G.chunk_state + r * cstride;
         const int32_t *perm = G.perm + (int64_t)r * G.perm_stride;
-        if (G.kind == MULTI_LDS) {
-            const LdsState st{codes, 1u << G.shift, G.shift};
-            sweep_run<LdsState, uint32_t, true, true>(
-                st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
-                claims, handoff, perm, (uint32_t *)G.records + (int64_t)r * G.rec_stride,
-                G.span_step + r, cs);
-        } else {
-            const GlobalState st{workspace + (size_t)blockIdx.x * slot_words};
-            if (G.kind == MULTI_HBM32)
-                sweep_run<GlobalState, uint32_t, true, true>(
-                    st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
-                    claims, handoff, perm,
-                    (uint32_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
-            else
-                sweep_run<GlobalState, uint64_t, true, true>(
-                    st, G.N, G.M, G.edges, G.sides, G.span_first != 0u, G.rounds,
-                    claims, handoff, perm,
-                    (uint64_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
-        }
+        auto run = [&](const auto &edges) {
+            if (G.kind == MULTI_LDS) {
+                const LdsState st{codes, 1u << G.shift, G.shift};
+                sweep_run<LdsState, uint32_t, true, true>(
+                    st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
+                    claims, handoff, perm, (uint32_t *)G.records + (int64_t)r * G.rec_stride,
+                    G.span_step + r, cs);
+            } else {
+                const GlobalState st{workspace + (size_t)blockIdx.x * slot_words};
+                if (G.kind == MULTI_HBM32)
+                    sweep_run<GlobalState, uint32_t, true, true>(
+                        st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
+                        claims, handoff, perm,
+                        (uint32_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
+                else
+                    sweep_run<GlobalState, uint64_t, true, true>(
+                        st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
+                        claims, handoff, perm,
+                        (uint64_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
+            }
+        };
+        if (G.lattice.L)
+            run(G.lattice);
+        else
+            run(EdgeList{G.edges});
         __syncthreads();
     }
 }
@@ -3298,8 +3352,10 @@ extern "C" int pc_sweep_workspace_size(uint32_t num_nodes, uint32_t num_runs,
     return PC_OK;
 }
 
-extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
-                        const int32_t *d_edges, const uint8_t *d_node_sides,
+// The sweep launch for either edge source (pc_sweep, pc_sweep_lattice).
+template <class ES>
+static int sweep_launch(uint32_t num_nodes, uint32_t num_edges, ES edges,
+                        bool have_edges, const uint8_t *d_node_sides,
                         int32_t span_on_first_merge, const int32_t *d_perm,
                         int64_t perm_stride, uint32_t num_runs, void *d_records,
                         int64_t rec_stride, uint32_t *d_span_step,
@@ -3326,12 +3382,11 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         return pc_fail(PC_EINVAL, "pc_sweep: num_nodes must be positive");
     if (num_nodes > kSweepMaxNodes)
         return pc_fail(PC_EOVERFLOW, "pc_sweep: num_nodes > 2^28");
-    if (!d_span_step || (num_edges && (!d_edges || !d_perm || !d_records)))
+    if (!d_span_step || (num_edges && (!have_edges || !d_perm || !d_records)))
         return pc_fail(PC_EINVAL, "pc_sweep: NULL buffer");
     if (num_edges && (perm_stride < (int64_t)num_edges ||
                       rec_stride < (int64_t)num_edges))
         return pc_fail(PC_EINVAL, "pc_sweep: stride smaller than num_edges");
-    const int2 *edges = (const int2 *)d_edges;
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
     const bool split = sweep_split(num_nodes, use_lds, waves);
@@ -3340,7 +3395,7 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
         const size_t lds = split ? sweep_lds_bytes<true>(num_nodes)
                                  : sweep_lds_bytes<false>(num_nodes);
 #define PC_SWEEP_LDS(CH, SP, CS)                                                    \
-    sweep_lds_kernel<uint32_t, CH, SP><<<num_runs, threads, lds, s>>>(               \
+    sweep_lds_kernel<uint32_t, CH, SP, ES><<<num_runs, threads, lds, s>>>(               \
         num_nodes, num_edges, edges, d_node_sides, span_on_first_merge, d_perm,     \
         perm_stride, (uint32_t *)d_records, rec_stride, d_span_step, rounds, CS)
         if (d_chunk_state && split)
@@ -3368,7 +3423,7 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
     if (slots > kGlobalSlots)
         slots = kGlobalSlots;
 #define PC_SWEEP_GLOBAL(RT, CH, SP)                                            \
-    sweep_global_kernel<RT, CH, SP><<<(uint32_t)slots, threads, 0, s>>>(       \
+    sweep_global_kernel<RT, CH, SP, ES><<<(uint32_t)slots, threads, 0, s>>>(       \
         num_nodes, num_edges, num_runs, edges, d_node_sides,                   \
         span_on_first_merge, d_perm, perm_stride, (RT *)d_records, rec_stride, \
         d_span_step, (uint32_t *)d_workspace, rounds, d_chunk_state)
@@ -3391,6 +3446,51 @@ extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
 #undef PC_SWEEP_GLOBAL
     PC_LAUNCH_CHECK("sweep_global_kernel");
     return PC_OK;
+}
+
+extern "C" int pc_sweep(uint32_t num_nodes, uint32_t num_edges,
+                        const int32_t *d_edges, const uint8_t *d_node_sides,
+                        int32_t span_on_first_merge, const int32_t *d_perm,
+                        int64_t perm_stride, uint32_t num_runs, void *d_records,
+                        int64_t rec_stride, uint32_t *d_span_step,
+                        void *d_workspace, uint64_t workspace_bytes,
+                        uint32_t merge_rounds, uint32_t state_mode,
+                        uint64_t *d_chunk_state, void *stream)
+{
+    return sweep_launch(num_nodes, num_edges, EdgeList{(const int2 *)d_edges},
+                        d_edges != nullptr, d_node_sides, span_on_first_merge,
+                        d_perm, perm_stride, num_runs, d_records, rec_stride,
+                        d_span_step, d_workspace, workspace_bytes, merge_rounds,
+                        state_mode, d_chunk_state, stream);
+}
+
+static LatticeEdges lattice_edges(uint32_t length, bool tiled)
+{
+    const uint32_t d = 2u * length - 1u;
+    return LatticeEdges{length, d, 1.0f / (float)d, tiled ? 1u : 0u};
+}
+
+extern "C" int pc_sweep_lattice(uint32_t length, int32_t tiled,
+                                const uint8_t *d_node_sides,
+                                int32_t span_on_first_merge, const int32_t *d_perm,
+                                int64_t perm_stride, uint32_t num_runs,
+                                void *d_records, int64_t rec_stride,
+                                uint32_t *d_span_step, void *d_workspace,
+                                uint64_t workspace_bytes, uint32_t merge_rounds,
+                                uint32_t state_mode, uint64_t *d_chunk_state,
+                                void *stream)
+{
+    g_last_error[0] = 0;
+    if (length == 0 || length > 16384u)
+        return pc_fail(PC_EINVAL, "pc_sweep_lattice: length must be in 1..16384");
+    if (tiled && (length % 8u))
+        return pc_fail(PC_EINVAL, "pc_sweep_lattice: the tiled order needs a multiple of 8");
+    const uint32_t N = length * length, M = 2u * length * (length - 1u);
+    return sweep_launch(N, M, lattice_edges(length, tiled != 0), true, d_node_sides,
+                        span_on_first_merge, d_perm, perm_stride, num_runs,
+                        d_records, rec_stride, d_span_step, d_workspace,
+                        workspace_bytes, merge_rounds, state_mode, d_chunk_state,
+                        stream);
 }
 
 extern "C" int pc_chunk_scan(uint32_t num_nodes, uint32_t num_edges,
@@ -3982,14 +4082,22 @@ static int multi_plan(uint32_t num_graphs, const pc_sweep_graph *graphs,
         const uint32_t rounds = in.merge_rounds ? in.merge_rounds : g_merge_rounds;
         if (in.num_nodes == 0 || in.num_nodes > kSweepMaxNodes || rounds > 64)
             return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: bad sizes", g);
+        const uint32_t L = in.lattice_length;
+        if (L && ((uint64_t)L * L != in.num_nodes ||
+                  in.num_edges != 2u * L * (L - 1u) ||
+                  (in.num_nodes > kLdsMaxNodes && (L % 8u))))
+            return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u is not the lattice of "
+                           "side %u (tiled order needs a multiple of 8)", g, L);
         if (in.num_runs && in.num_edges &&
-            (!in.d_edges || !in.d_perm || !in.d_records || !in.d_span_step ||
+            ((!L && !in.d_edges) || !in.d_perm || !in.d_records || !in.d_span_step ||
              !in.d_chunk_state || in.perm_stride < (int64_t)in.num_edges ||
              in.rec_stride < (int64_t)in.num_edges))
             return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: bad buffers", g);
         if (in.num_runs && !in.d_span_step)
             return pc_fail(PC_EINVAL, "pc_sweep_multi: graph %u: no span output", g);
         G.edges = (const int2 *)in.d_edges;
+        G.lattice = L ? lattice_edges(L, in.num_nodes > kLdsMaxNodes)
+                      : LatticeEdges{0u, 1u, 1.0f, 0u};
         G.sides = in.d_node_sides;
         G.perm = in.d_perm;
         G.records = in.d_records;
@@ -4126,18 +4234,18 @@ extern "C" int pc_sweep_occupancy(uint32_t num_nodes, uint32_t state_mode,
             return pc_fail(PC_EINVAL, "pc_sweep_occupancy: too many nodes for LDS");
         if (split)
             PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &blocks, sweep_lds_kernel<uint32_t, false, true>, 128,
+                &blocks, sweep_lds_kernel<uint32_t, false, true, EdgeList>, 128,
                 sweep_lds_bytes<true>(num_nodes)));
         else
             PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &blocks, sweep_lds_kernel<uint32_t, false, false>, 64,
+                &blocks, sweep_lds_kernel<uint32_t, false, false, EdgeList>, 64,
                 sweep_lds_bytes<false>(num_nodes)));
     } else if (split) {
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_global_kernel<uint32_t, false, true>, 128, 0));
+            &blocks, sweep_global_kernel<uint32_t, false, true, EdgeList>, 128, 0));
     } else {
         PC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, sweep_global_kernel<uint32_t, false, false>, 64, 0));
+            &blocks, sweep_global_kernel<uint32_t, false, false, EdgeList>, 64, 0));
     }
     *blocks_per_cu = blocks;
     return PC_OK;

@@ -217,6 +217,46 @@ def test_pipeline_prefetch_matches_single_launch(gpu, L, after):
         assert np.array_equal(pipe.perm.cpu().numpy(), res.perm.cpu().numpy()), k
 
 
+@pytest.mark.parametrize('L,relabel', [(1, False), (2, False), (3, False),
+                                       (8, False), (16, False), (37, False),
+                                       (128, False), (8, True), (136, True),
+                                       (264, True), (130, False)])
+def test_lattice_edges_equal_edge_list(gpu, L, relabel):
+    """pc_sweep_lattice (endpoints computed from the edge index, row-major
+    or tiled node order) gives pc_sweep's records, span steps and chunk
+    states over the closed-form lattice's edge array, in LDS and HBM state
+    (L = 136, 264 tiled; L = 130 row-major in HBM)."""
+    ga = _graph.square_lattice(L)
+    eng = BondEngine(ga, relabel=relabel)
+    assert eng.relabelled == relabel
+    assert eng.lattice_length == L and eng.lattice_tiled == relabel
+    assert eng.lattice_sweep == (L * L > 16384)
+    runs = 6 if L < 200 else 3
+    perm = eng.device_permutations(31, runs)
+    eng.lattice_sweep = True
+    lat = eng.sweep(perm, chunks=True)
+    eng.lattice_sweep = False
+    ref = eng.sweep(perm, chunks=True)
+    M = eng.M   # L = 1 has no edges: nothing is written
+    np.testing.assert_array_equal(lat.records.cpu().numpy()[:, :M],
+                                  ref.records.cpu().numpy()[:, :M])
+    np.testing.assert_array_equal(lat.span_step.cpu().numpy(), ref.span_step.cpu().numpy())
+    if M:
+        np.testing.assert_array_equal(lat.chunk_state.cpu().numpy(),
+                                      ref.chunk_state.cpu().numpy())
+
+
+def test_lattice_sweep_rejects_bad_sizes(gpu):
+    from pypercolate_amd import _native as nat
+    lib = nat.load_library()
+    one = gpu.zeros(8, dtype=gpu.int32, device='cuda')
+    args = (None, 0, nat.ptr(one), 1, 1, nat.ptr(one), 1, nat.ptr(one),
+            None, 0, 0, 0, None, None)
+    assert lib.pc_sweep_lattice(0, 0, *args) == -1
+    assert lib.pc_sweep_lattice(12, 1, *args) == -1       # tiles need L % 8 == 0
+    assert lib.pc_sweep_lattice(20000, 0, *args) == -1
+
+
 def test_pipeline_bit_exact_mode_matches_oracle(gpu):
     """The pipeline with device RandomState permutations (seeds = run
     index, the L = 16 configuration) gives the oracle's exact sums over
