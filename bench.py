@@ -126,6 +126,11 @@ def large_lattice(length, runs, seed, steps=2, warmup=1):
         'sums_check': ok,
     }
     info['roofline_sweep']['frac'] = info['roofline_sweep']['achieved'] / HBM_PEAK_GBS
+    per_add = profiled_traffic_per_add(length, runs)
+    info['roofline_sweep']['traffic'] = (per_add * adds if per_add is not None
+                                         else None)
+    info['roofline_sweep']['edges'] = ('computed from the edge index'
+                                       if eng.lattice_sweep else 'edge array')
     sums = (pipe.sum1, pipe.sum2)
     del pipe
     torch.cuda.empty_cache()
