@@ -358,16 +358,12 @@ class Pipeline(object):
             # measured slower with the current one too)
             cus = torch.cuda.get_device_properties(dev).multi_processor_count
             sub_batch = max(eng.occupancy(), 1) * cus
-            if seeds is not None:
-                # RandomState permutations: one wavefront per run walks its
-                # shuffle in sequence, so a launch needs many runs per SIMD
-                # to hide its latencies; they, not the sweeps, set the pace
-                sub_batch *= 4
         self.sub = max(1, min(int(sub_batch), self.R))
-        # Philox permutations of every run in one launch before the sweeps
-        # (the shuffle needs 72 KiB of LDS per run, so beside the sweeps it
-        # waits for LDS they hold) or per sub-batch beside the sweeps
-        self.permute_ahead = (seeds is None) if permute_ahead is None else bool(permute_ahead)
+        # permutations of every run in one launch before the sweeps (both
+        # kinds need most of a CU's LDS per run, so beside the sweeps they
+        # wait for LDS the sweeps hold: 11.9 ms per step of 10^4 RandomState
+        # runs at L = 128 against 18.0 per sub-batch), or per sub-batch
+        self.permute_ahead = True if permute_ahead is None else bool(permute_ahead)
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
         self.prefetch = bool(prefetch) and self.permute_ahead and seeds is None
         self.prefetch_after = max(0, int(prefetch_after))

@@ -834,7 +834,8 @@ __device__ __forceinline__ void mt_twist_range(uint32_t *mt, uint32_t lo,
         const uint32_t k = k0 + threadIdx.x;
         uint32_t v = 0;
         if (k < hi)
-            v = mt_next_word(mt[k], mt[(k + 1u) % MT_N], mt[(k + MT_SHIFT) % MT_N]);
+            v = mt_next_word(mt[k], mt[k + 1u == MT_N ? 0u : k + 1u],
+                             mt[k < MT_N - MT_SHIFT ? k + MT_SHIFT : k - (MT_N - MT_SHIFT)]);
         if (k < hi)
             mt[k] = v;
     }
@@ -930,6 +931,336 @@ __global__ __launch_bounds__(64) void permute_randomstate_kernel(
         for (uint32_t k = lane; k < n; k += 64u)
             out[k] = (int32_t)arr16[k];
     }
+}
+
+// The same permutation in two parallel phases (runs of at most
+// RS_APPLY_MAX edges).  The shuffle's swap targets depend on the MT19937
+// stream alone, not on the array: phase one draws them (one wavefront per
+// run) and stores the target k_i of step i in the output row; phase two
+// applies all swaps at once.  Steps run i = n-1 .. 1 and step i writes only
+// positions i and k_i <= i, so no later step writes position i again and
+// the result at position i >= 1 is the value position k_i holds just before
+// step i.  The writers of a position p before step i are the steps s > i
+// with k_s = p, the last of them being the smallest; the value such a step
+// s brings to p is the one position s holds just before step s.  With
+//   F(p) = the smallest step s > p with k_s = p (none possible),
+//   V(p) = the value position p holds just before step p
+//        = V(F(p)), or p itself when F(p) does not exist,
+// V is the root of p's chain p -> F(p) -> F(F(p)) ... (F only increases),
+// the result at i >= 1 is V(s) for the smallest step s > i with k_s = k_i,
+// or k_i itself without one, and the result at 0 is V(0).
+#define RS_APPLY_MAX 32768u
+#define RS_APPLY_T 1024u
+
+__device__ __forceinline__ uint32_t all_ones_below(uint32_t i)
+{
+    i |= i >> 1;
+    i |= i >> 2;
+    i |= i >> 4;
+    i |= i >> 8;
+    i |= i >> 16;
+    return i;
+}
+
+// Phase one.  The wave tempers 64 state words at a time (lane l holds word
+// base + l) and settles the whole block at once where it can: for steps
+// i, i-1, ..., i-63 under one mask m, the word of lane l serves step
+// i - (accepted words below l), so x = word & m is accepted for sure when
+// x <= i - 63, rejected for sure when x > i, and only the few lanes in
+// between are decided one by one in lane order from the count of accepted
+// lanes below them.  Blocks where the mask changes, or with fewer than 64
+// steps left, take the draws one at a time.
+__global__ __launch_bounds__(64) void randomstate_targets_kernel(
+    const uint32_t *__restrict__ seeds, uint32_t n, int32_t *__restrict__ perm,
+    int64_t stride)
+{
+    __shared__ uint32_t mt[MT_N];
+    const uint32_t lane = threadIdx.x;
+    int32_t *out = perm + (int64_t)blockIdx.x * stride;
+    if (lane == 0) {
+        uint32_t x = seeds[blockIdx.x];
+        mt[0] = x;
+        for (uint32_t i = 1; i < MT_N; ++i) {
+            x = 1812433253u * (x ^ (x >> 30)) + i;
+            mt[i] = x;
+        }
+        if (n)
+            out[0] = 0;
+    }
+    __syncthreads();
+    uint32_t pos = MT_N;
+    uint32_t i = n ? n - 1u : 0u;
+    while (i >= 1u) {
+        if (pos == MT_N) {
+            __syncthreads();
+            mt_twist(mt);
+            __syncthreads();
+            pos = 0;
+        }
+        const uint32_t nw = min(64u, MT_N - pos);
+        const uint32_t w = lane < nw ? mt_temper(mt[pos + lane]) : 0u;
+        const uint32_t m = all_ones_below(i);
+        if (i >= 64u && all_ones_below(i - 63u) == m) {
+            const uint32_t x = w & m;
+            const bool valid = lane < nw;
+            uint64_t acc = __ballot(valid && x + 63u <= i);
+            uint64_t amb = __ballot(valid && x <= i && x + 63u > i);
+            while (amb) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(amb);
+                const uint32_t below = (uint32_t)__popcll(acc & ((1ull << l) - 1ull));
+                if (read_lane(x, l) + below <= i)
+                    acc |= 1ull << l;
+                amb &= amb - 1ull;
+            }
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(acc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)acc, 0u));
+            if ((acc >> lane) & 1ull)
+                out[i - below] = (int32_t)x;
+            i -= (uint32_t)__popcll(acc);
+            pos += nw;
+        } else {
+            uint32_t q = 0;
+            while (q < nw && i >= 1u) {
+                const uint32_t x = read_lane(w, q) & all_ones_below(i);
+                ++q;
+                if (x <= i) {
+                    if (lane == 0u)
+                        out[i] = (int32_t)x;
+                    --i;
+                }
+            }
+            pos += q;
+        }
+    }
+}
+
+// smallest of the list entries ent[b..e) above thr, or 0xFFFF (entries are
+// steps, 1 <= s < n <= 32768); four entries are read per round
+__device__ __forceinline__ uint32_t rs_list_min_above(const uint16_t *ent,
+                                                      uint32_t b, uint32_t e,
+                                                      uint32_t thr)
+{
+    uint32_t best = 0xFFFFu;
+    for (uint32_t a = b; a < e; a += 4u) {
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u)
+            x[u] = ent[min(a + u, e - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u)
+            best = x[u] > thr ? min(best, x[u]) : best;
+    }
+    return best;
+}
+
+// Phase two, one workgroup of RS_APPLY_T threads per run of n <= RS_APPLY_MAX
+// edges, in place over the targets.  LDS holds 16-bit list cursors of the
+// positions in pairs and the lists of steps per target position (a counting
+// sort: counts, exclusive scan, placement; unordered within a list); every
+// thread owns indices j = tid + q * RS_APPLY_T and scans the lists of k_j
+// and of j for the smallest step above j.  The list area is then reused for
+// V, which pointer jumping (V(p) <- V(V(p)) until nothing changes, over
+// position pairs) takes to the chain roots.  The loads of independent items
+// are issued together before any is waited for.  Dynamic LDS:
+// rs_apply_lds_bytes(n).
+// A copy of x the compiler cannot relate to x: predicates computed from it in
+// one phase are not kept alive (as 32 lane masks) for a later phase.
+__device__ __forceinline__ uint32_t opaque(uint32_t x)
+{
+    asm volatile("" : "+s"(x));
+    return x;
+}
+__device__ __forceinline__ uint32_t opaque_v(uint32_t x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+__global__ __launch_bounds__(RS_APPLY_T) void randomstate_apply_kernel(
+    uint32_t n, int32_t *__restrict__ perm, int64_t stride)
+{
+    constexpr uint32_t T = RS_APPLY_T, PER = RS_APPLY_MAX / RS_APPLY_T;
+    constexpr uint32_t PAIRS = PER / 2u, WAVES = T / 64u, LOOKUP = 0x8000u;
+    constexpr uint32_t GROUP = 4u;
+    extern __shared__ uint32_t rsd[];
+    const uint32_t words = (n + 2u) / 2u;   // 16-bit cursors of positions 0..n
+    uint32_t *cur = rsd;
+    uint16_t *ent = (uint16_t *)(rsd + words);
+    uint32_t *V2 = rsd + words;             // V of positions 2w, 2w + 1
+    const uint16_t *V = ent;
+    __shared__ uint32_t wsum[WAVES];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    int32_t *row = perm + (int64_t)blockIdx.x * stride;
+    if (n == 0u)
+        return;
+    const uint32_t npairs = (n + 1u) / 2u;
+    uint32_t k[PER];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = tid + q * T;
+        const uint32_t v = (uint32_t)row[min(j, n - 1u)];
+        k[q] = (j >= 1u && j < n) ? v : 0u;
+    }
+    for (uint32_t w = tid; w < words; w += T)
+        cur[w] = 0u;
+    __syncthreads();
+    // steps per target position
+    {
+        const uint32_t nn = opaque(n), t0 = opaque_v(tid);
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q) {
+            const uint32_t j = t0 + q * T;
+            if (j >= 1u && j < nn)
+                atomicAdd(&cur[k[q] >> 1], 1u << ((k[q] & 1u) * 16u));
+        }
+    }
+    __syncthreads();
+    // exclusive prefix: cur[p] = first slot of position p's list.  Wave w
+    // scans words [w * chunk, (w + 1) * chunk), 64 consecutive words per
+    // round, after one pass for the wave totals.
+    {
+        const uint32_t chunk = ((words + WAVES - 1u) / WAVES + 63u) & ~63u;
+        const uint32_t w0 = min(wave * chunk, words), w1 = min(w0 + chunk, words);
+        uint32_t tot = 0;
+        for (uint32_t w = w0 + lane; w < w1; w += 64u) {
+            const uint32_t x = cur[w];
+            tot += (x & 0xFFFFu) + (x >> 16);
+        }
+        tot = read_lane(wave_scan_add(tot), 63u);
+        if (lane == 0u)
+            wsum[wave] = tot;
+        __syncthreads();
+        uint32_t carry = 0;
+        for (uint32_t w = 0; w < wave; ++w)
+            carry += wsum[w];
+        for (uint32_t b = w0; b < w1; b += 64u) {
+            const uint32_t w = b + lane;
+            const uint32_t x = w < w1 ? cur[w] : 0u;
+            const uint32_t pair = (x & 0xFFFFu) + (x >> 16);
+            const uint32_t inc = wave_scan_add(pair);
+            const uint32_t a0 = carry + inc - pair, a1 = a0 + (x & 0xFFFFu);
+            if (w < w1)
+                cur[w] = a0 | (a1 << 16);
+            carry += read_lane(inc, 63u);
+        }
+    }
+    __syncthreads();
+    // place the steps; afterwards cursor p is the end of p's list, which is
+    // where p + 1's list starts
+    const uint32_t np = opaque(n), tp = opaque_v(tid);
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+        const uint32_t j = tp + q * T;
+        if (j >= 1u && j < np) {
+            const uint32_t sh = (k[q] & 1u) * 16u;
+            const uint32_t at = (atomicAdd(&cur[k[q] >> 1], 1u << sh) >> sh) & 0xFFFFu;
+            ent[at] = (uint16_t)j;
+        }
+    }
+    __syncthreads();
+    // per index, packed: low half the result, or LOOKUP | s for "V(s)";
+    // high half F(j), or j without one (all values are below 2^15).  The
+    // list bounds of a group of indices are read before the lists.
+    auto half = [&](uint32_t x, uint32_t p) { return (x >> ((p & 1u) * 16u)) & 0xFFFFu; };
+    uint32_t rf[PER];
+    const uint32_t nr = opaque(n), tr = opaque_v(tid);
+#pragma unroll
+    for (uint32_t g = 0; g < PER; g += GROUP) {
+        uint32_t se[GROUP], sb[GROUP], fe[GROUP], fb[GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < GROUP; ++u) {
+            const uint32_t j = min(tr + (g + u) * T, nr - 1u), p = k[g + u];
+            se[u] = cur[p >> 1];
+            sb[u] = cur[p ? (p - 1u) >> 1 : 0u];     // unused for p = 0
+            fe[u] = cur[j >> 1];
+            fb[u] = cur[j ? (j - 1u) >> 1 : 0u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < GROUP; ++u) {
+            const uint32_t q = g + u, j = tr + q * T;
+            uint32_t r = LOOKUP, f = j;
+            if (j < nr) {
+                const uint32_t p = k[q];
+                if (j) {
+                    const uint32_t b = p ? half(sb[u], p - 1u) : 0u, e = half(se[u], p);
+                    const uint32_t s = rs_list_min_above(ent, b, e, j);
+                    r = s == 0xFFFFu ? p : (LOOKUP | s);
+                }
+                const uint32_t b = j ? half(fb[u], j - 1u) : 0u, e = half(fe[u], j);
+                const uint32_t s = rs_list_min_above(ent, b, e, j);
+                f = s == 0xFFFFu ? j : s;
+            }
+            rf[q] = r | (f << 16);
+        }
+    }
+    __syncthreads();
+    // the results move to the cursor area, V (initialised with F) to the
+    // list area
+    uint16_t *res = (uint16_t *)cur;
+    {
+        uint16_t *Vw = ent;
+        const uint32_t nv = opaque(n), tv = opaque_v(tid);
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q) {
+            const uint32_t j = tv + q * T;
+            if (j < nv) {
+                res[j] = (uint16_t)rf[q];
+                Vw[j] = (uint16_t)(rf[q] >> 16);
+            }
+        }
+    }
+    __syncthreads();
+    // pointer jumping over the pairs; a concurrent update only moves a
+    // pointer further up its own chain, and a round without a change leaves
+    // every pointer at a root.  An odd n's last pair has a padding half,
+    // kept as it is.
+    for (;;) {
+        uint32_t v[PAIRS], y0[PAIRS], y1[PAIRS];
+        const uint32_t tj = opaque_v(tid);
+#pragma unroll
+        for (uint32_t h = 0; h < PAIRS; ++h)
+            v[h] = V2[min(tj + h * T, npairs - 1u)];
+#pragma unroll
+        for (uint32_t h = 0; h < PAIRS; ++h) {
+            y0[h] = V[v[h] & 0xFFFFu];
+            y1[h] = V[min(v[h] >> 16, n - 1u)];
+        }
+        bool changed = false;
+#pragma unroll
+        for (uint32_t h = 0; h < PAIRS; ++h) {
+            const uint32_t w = tj + h * T;
+            const uint32_t hi = 2u * w + 1u < n ? y1[h] : v[h] >> 16;
+            const uint32_t nv = y0[h] | (hi << 16);
+            const bool c = w < npairs && nv != v[h];
+            if (c)
+                V2[w] = nv;
+            changed |= c;
+        }
+        if (!__syncthreads_or(changed))
+            break;
+    }
+    const uint32_t tf = opaque_v(tid);
+#pragma unroll
+    for (uint32_t g = 0; g < PER; g += GROUP) {
+        uint32_t r[GROUP], val[GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < GROUP; ++u)
+            r[u] = res[min(tf + (g + u) * T, n - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < GROUP; ++u)
+            val[u] = (r[u] & LOOKUP) ? (uint32_t)V[min(r[u] & ~LOOKUP, n - 1u)] : r[u];
+#pragma unroll
+        for (uint32_t u = 0; u < GROUP; ++u) {
+            const uint32_t j = tf + (g + u) * T;
+            if (j < n)
+                row[j] = (int32_t)val[u];
+        }
+    }
+}
+
+static size_t rs_apply_lds_bytes(uint32_t n)
+{
+    return (size_t)((n + 2u) / 2u) * 4u + (((size_t)n * 2u + 3u) & ~(size_t)3u);
 }
 
 // ---------------------------------------------------------------------------
@@ -4320,6 +4651,16 @@ extern "C" int pc_permute_randomstate(const uint32_t *d_seeds, uint32_t num_runs
     if (num_edges > 0x7fffffffu)
         return pc_fail(PC_EOVERFLOW, "pc_permute_randomstate: num_edges >= 2^31");
     hipStream_t s = as_stream(stream);
+    if (num_edges <= RS_APPLY_MAX) {
+        // targets, then all swaps at once (in place in the output rows)
+        randomstate_targets_kernel<<<num_runs, 64, 0, s>>>(d_seeds, num_edges, d_perm,
+                                                             perm_stride);
+        PC_LAUNCH_CHECK("randomstate_targets_kernel");
+        randomstate_apply_kernel<<<num_runs, RS_APPLY_T, rs_apply_lds_bytes(num_edges), s>>>(
+            num_edges, d_perm, perm_stride);
+        PC_LAUNCH_CHECK("randomstate_apply_kernel");
+        return PC_OK;
+    }
     if (num_edges <= MT_LDS_MAX) {
         const size_t lds = ((size_t)num_edges * sizeof(uint16_t) + 3u) & ~(size_t)3u;
         permute_randomstate_kernel<true><<<num_runs, 64, lds, s>>>(

@@ -257,24 +257,29 @@ def test_lattice_sweep_rejects_bad_sizes(gpu):
     assert lib.pc_sweep_lattice(20000, 0, *args) == -1
 
 
-def test_pipeline_bit_exact_mode_matches_oracle(gpu):
+@pytest.mark.parametrize('ahead', [True, False])
+def test_pipeline_bit_exact_mode_matches_oracle(gpu, ahead):
     """The pipeline with device RandomState permutations (seeds = run
     index, the L = 16 configuration) gives the oracle's exact sums over
-    numpy's permutations."""
+    numpy's permutations, drawn all before the sweeps or per sub-batch."""
     from pypercolate_amd._engine import Pipeline
     eng = BondEngine(_graph.square_lattice(16))
     g = oracle.percolation_graph_arrays(spanning_2d_grid(16), True)
     tr = oracle.bond_runs(g, oracle.randomstate_permutations(range(40), g['num_edges']))
     ref1, ref2 = oracle.microcanonical_sums(tr)
-    pipe = Pipeline(eng, 40, seeds=np.arange(40), sub_batch=16)
+    pipe = Pipeline(eng, 40, seeds=np.arange(40), sub_batch=16, permute_ahead=ahead)
     s1, s2 = host_sums(*pipe.run())
     assert (s1 == ref1).all() and (s2 == ref2).all()
 
 
-@pytest.mark.parametrize('M', [1, 2, 12, 480, 32512, 70000])
+@pytest.mark.parametrize('M', [1, 2, 3, 12, 63, 64, 65, 129, 480, 4097, 32512,
+                               32768, 32769, 65536, 70000])
 def test_randomstate_permutations_bit_exact(gpu, M):
-    """Device MT19937 + legacy shuffle == numpy RandomState(seed).permutation
-    (LDS path for M <= 65536, global path above)."""
+    """Device MT19937 + legacy shuffle == numpy RandomState(seed).permutation:
+    up to 32768 edges the swap targets drawn first (64-word blocks settled
+    at once) and the swaps applied all at once (per-position step lists,
+    pointer jumping); above, the shuffle walked in sequence (LDS array up
+    to 65536 edges, the output row above)."""
     eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
         [np.arange(M), np.arange(1, M + 1)], axis=1)))
     seeds = [0, 1, 42, 12345, 4294967295, 2 ** 31 + 7]
