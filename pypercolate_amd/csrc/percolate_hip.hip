@@ -952,24 +952,15 @@ __global__ __launch_bounds__(64) void permute_randomstate_kernel(
 #define RS_APPLY_MAX 32768u
 #define RS_APPLY_T 1024u
 
-__device__ __forceinline__ uint32_t all_ones_below(uint32_t i)
-{
-    i |= i >> 1;
-    i |= i >> 2;
-    i |= i >> 4;
-    i |= i >> 8;
-    i |= i >> 16;
-    return i;
-}
-
 // Phase one.  The wave tempers 64 state words at a time (lane l holds word
-// base + l) and settles the whole block at once where it can: for steps
-// i, i-1, ..., i-63 under one mask m, the word of lane l serves step
-// i - (accepted words below l), so x = word & m is accepted for sure when
-// x <= i - 63, rejected for sure when x > i, and only the few lanes in
+// base + l) and settles the whole block at once: for steps i, i-1, ... under
+// one mask m (the smallest all-ones mask >= i), the word of lane l serves
+// step i - (accepted words below l), so x = word & m is accepted for sure
+// when x <= i - 63, rejected for sure when x > i, and only the few lanes in
 // between are decided one by one in lane order from the count of accepted
-// lanes below them.  Blocks where the mask changes, or with fewer than 64
-// steps left, take the draws one at a time.
+// lanes below them.  The mask holds for c = i - m/2 more steps; when the
+// block accepts c words or more, the lanes after the c-th accepted one are
+// settled again under the next mask.
 __global__ __launch_bounds__(64) void randomstate_targets_kernel(
     const uint32_t *__restrict__ seeds, uint32_t n, int32_t *__restrict__ perm,
     int64_t stride)
@@ -999,10 +990,12 @@ __global__ __launch_bounds__(64) void randomstate_targets_kernel(
         }
         const uint32_t nw = min(64u, MT_N - pos);
         const uint32_t w = lane < nw ? mt_temper(mt[pos + lane]) : 0u;
-        const uint32_t m = all_ones_below(i);
-        if (i >= 64u && all_ones_below(i - 63u) == m) {
+        uint32_t l0 = 0;
+        do {
+            const uint32_t m = 0xFFFFFFFFu >> __builtin_clz(i);
+            const uint32_t c = i - (m >> 1);
             const uint32_t x = w & m;
-            const bool valid = lane < nw;
+            const bool valid = lane >= l0 && lane < nw;
             uint64_t acc = __ballot(valid && x + 63u <= i);
             uint64_t amb = __ballot(valid && x <= i && x + 63u > i);
             while (amb) {
@@ -1014,23 +1007,25 @@ __global__ __launch_bounds__(64) void randomstate_targets_kernel(
             }
             const uint32_t below = __builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(acc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)acc, 0u));
-            if ((acc >> lane) & 1ull)
-                out[i - below] = (int32_t)x;
-            i -= (uint32_t)__popcll(acc);
-            pos += nw;
-        } else {
-            uint32_t q = 0;
-            while (q < nw && i >= 1u) {
-                const uint32_t x = read_lane(w, q) & all_ones_below(i);
-                ++q;
-                if (x <= i) {
-                    if (lane == 0u)
-                        out[i] = (int32_t)x;
-                    --i;
-                }
+            const bool mine = (acc >> lane) & 1ull;
+            const uint32_t cnt = (uint32_t)__popcll(acc);
+            uint32_t used;
+            if (cnt >= c) {
+                // the c-th accepted lane takes the mask's last step
+                const uint32_t last = (uint32_t)__builtin_ctzll(__ballot(mine && below == c - 1u));
+                if (mine && below < c)
+                    out[i - below] = (int32_t)x;
+                used = c;
+                l0 = last + 1u;
+            } else {
+                if (mine)
+                    out[i - below] = (int32_t)x;
+                used = cnt;
+                l0 = nw;
             }
-            pos += q;
-        }
+            i -= used;
+        } while (l0 < nw && i >= 1u);
+        pos += nw;
     }
 }
 
