@@ -2214,28 +2214,76 @@ __global__ __launch_bounds__(64) void micro_sums_kernel(
         acc1[6] = m1_4;
         acc2[6] = m2_4;
     } else {
-    for (uint32_t r = r0; r < r1; ++r) {
-        const RunState<WIDE> st0 = load_state<WIDE>(
-            chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS);
-        const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
-        const RunState<WIDE> st = advance<WIDE>(st0, Deltas<WIDE>(rec));
-        MT x[7];
-        observables<WIDE>(st, N, s >= span_step[r], x);
+        // N <= 2^21 (the sums' limit) and at most 2^16 runs per group: S3 is
+        // below N^3 <= 2^63 (its deltas and scan stay in 64 bits), max,
+        // #clusters - 1 and N - max are below 2^21 (group sums of squares
+        // below 2^58), moment 2 below 2^42 (its square below 2^84), moment 3
+        // below 2^63 (square below 2^126) and moment 4 below 2^84 (square
+        // below 2^168).  Only moments 3 and 4 need 192-bit sums of squares.
+        uint64_t n1[5] = {0, 0, 0, 0, 0}; // sum x: max, span, #clusters - 1, N - max, moment 2
+        uint64_t n2[4] = {0, 0, 0, 0};    // sum x^2 of the first four
+        u128 m2_2 = 0, m1_3 = 0, m1_4 = 0;
+        u128 m2_3 = 0, m2_4 = 0;          // low 128 bits of sum x^2, moments 3 and 4
+        uint64_t t2_3 = 0, t2_4 = 0;      // their top words
+        for (uint32_t r = r0; r < r1; ++r) {
+            const uint64_t *cs = chunk_state + ((size_t)r * C + c) * PC_STATE_WORDS;
+            const uint64_t mx0 = cs[0], merges0 = cs[1], s2_0 = cs[2], s3_0 = cs[3];
+            const u128 s4_0 = (u128)cs[5] | ((u128)cs[6] << 64);
+            const RecT rec = valid ? records[(int64_t)r * rec_stride + q] : (RecT)0;
+            uint32_t a, b;
+            unpack_record(rec, a, b);
+            const bool merge = rec != 0;
+            const uint64_t ab = (uint64_t)a * b;
+            const uint64_t aa = (uint64_t)a * a, bb = (uint64_t)b * b;
+            const uint32_t dmax = merge ? a + b : 0u;
+            const uint64_t d2 = 2u * ab;
+            const uint64_t d3 = 3u * ab * (uint64_t)(a + b);
+            const u128 d4 = (u128)(2u * ab) * (u128)(2u * aa + 3u * ab + 2u * bb);
+            const uint64_t mx = umax64(mx0, (uint64_t)wave_scan_max(dmax));
+            const uint64_t merges = merges0 + wave_scan_add(merge ? 1u : 0u);
+            const uint64_t s2 = s2_0 + wave_scan_add(d2);
+            const uint64_t s3 = s3_0 + wave_scan_add(d3);
+            const u128 s4 = s4_0 + wave_scan_add(d4);
+            const uint64_t m2 = mx * mx;
+            const uint64_t x[4] = {mx, s >= span_step[r] ? 1u : 0u,
+                                   (uint64_t)N - merges - 1u, (uint64_t)N - mx};
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            acc1[k] += (u128)x[k];
-            if constexpr (WIDE) {
+            for (int k = 0; k < 4; ++k) {
+                n1[k] += x[k];
+                n2[k] += x[k] * x[k];
+            }
+            const uint64_t x4 = s2 - m2, x5 = s3 - m2 * mx;
+            const u128 x6 = s4 - (u128)m2 * m2;
+            n1[4] += x4;
+            m2_2 += (u128)x4 * x4;
+            m1_3 += x5;
+            {
+                const u128 sq = (u128)x5 * x5, before = m2_3;
+                m2_3 += sq;
+                t2_3 += m2_3 < before ? 1u : 0u;
+            }
+            m1_4 += x6;
+            {
                 uint64_t w0, w1, w2;
-                square_u192((u128)x[k], w0, w1, w2);
-                const u128 low = (u128)w0 | ((u128)w1 << 64);
-                const u128 before = acc2[k];
-                acc2[k] += low;
-                acc2_top[k] += w2 + (acc2[k] < before ? 1u : 0u);
-            } else {
-                acc2[k] += (u128)x[k] * (u128)x[k];
+                square_u192(x6, w0, w1, w2);
+                const u128 before = m2_4;
+                m2_4 += (u128)w0 | ((u128)w1 << 64);
+                t2_4 += w2 + (m2_4 < before ? 1u : 0u);
             }
         }
-    }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc1[k] = n1[k];
+            acc2[k] = n2[k];
+        }
+        acc1[4] = n1[4];
+        acc2[4] = m2_2;
+        acc1[5] = m1_3;
+        acc2[5] = m2_3;
+        acc2_top[5] = t2_3;
+        acc1[6] = m1_4;
+        acc2[6] = m2_4;
+        acc2_top[6] = t2_4;
     }
 
     if (valid) {
