@@ -169,7 +169,11 @@ int pc_permute_philox_workspace_size(uint32_t num_edges, uint32_t num_runs,
  * Bit-exact device permutations: d_perm[r * perm_stride + i] =
  * numpy.random.RandomState(d_seeds[r]).permutation(num_edges)[i], the
  * permutation the reference draws per run (percolate/hpc.py:195,206), from
- * MT19937 and numpy's legacy shuffle on the GPU (one run per wavefront).
+ * MT19937 and numpy's legacy shuffle on the GPU.  Up to 32768 edges in two
+ * kernels (the swap targets of every step from one wavefront per run, then
+ * every swap at once from one workgroup of 1024 threads and 4 bytes of LDS
+ * per edge); above, the swaps in sequence, one run per wavefront.  The
+ * output row is also the work space between the two kernels.
  */
 int pc_permute_randomstate(const uint32_t *d_seeds, uint32_t num_runs,
                            uint32_t num_edges, int32_t *d_perm,
