@@ -508,10 +508,12 @@ def reference_class_baseline(length, single_runs, runs_per_proc):
 
 def bitexact_mode(eng, R, steps, start, stop):
     """The L = 128 pipeline with numpy RandomState(r).permutation(M) for run r
-    drawn on the device (MT19937 + the legacy shuffle)."""
+    drawn on the device (MT19937 + the legacy shuffle), each step drawing
+    the next step's permutations beside its last two sweeps and reductions
+    (the headline loop draws them beside the last reductions)."""
     import torch
     from pypercolate_amd._engine import Pipeline
-    exact = Pipeline(eng, R, seeds=np.arange(R))
+    exact = Pipeline(eng, R, seeds=np.arange(R), prefetch=True, prefetch_after=2)
     exact.run()
     torch.cuda.synchronize()
     start.record()
@@ -524,7 +526,8 @@ def bitexact_mode(eng, R, steps, start, stop):
     res = {'value': float(R) * eng.M / (ms / 1e3), 'unit': 'bond additions/s',
            'ms_per_step': ms,
            'permutations': 'numpy RandomState(r).permutation(M), r = 0..{}, '
-                           'on the device (MT19937 + legacy shuffle)'.format(R - 1),
+                           'on the device (MT19937 + legacy shuffle), the next '
+                           'step\'s drawn beside the last two sweeps'.format(R - 1),
            'sums_check': bool(int(s1[0, eng.M, 0]) == R * eng.N and
                               int(s1[1, eng.M, 0]) == R)}
     del exact

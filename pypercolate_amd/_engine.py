@@ -327,13 +327,17 @@ class Pipeline(object):
     with seeds= (R integers) numpy RandomState(seeds[r]).permutation(M),
     bit-identical to the reference's draw, on the device.
 
-    prefetch (Philox permutations drawn ahead): each run() also draws the
+    prefetch (permutations drawn ahead): each run() also draws the
     permutations of the next run() into a second buffer, on the caller's
     stream once the sweep of sub-batch ``-1 - prefetch_after`` has ended,
     so that they fill the CUs the last sweeps and the reductions leave free;
-    the next run() starts with its sweeps.  The first run() draws its own.
-    Every run() still draws one batch of permutations, the same ones (the
-    permutations depend only on the seed and the run index)."""
+    the next run() starts with its sweeps (RandomState permutations too:
+    10.4 ms per step of 10^4 runs at L = 128 with prefetch_after = 2,
+    against 11.1 without; started earlier, they stretch the sweeps by about
+    as much as they hide).  The
+    first run() draws its own.  Every run() still draws one batch of
+    permutations, the same ones (the permutations depend only on the seeds,
+    or the seed and the run index)."""
 
     def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None,
                  permute_ahead=None, prefetch=False, prefetch_after=1):
@@ -365,7 +369,7 @@ class Pipeline(object):
         # runs at L = 128 against 18.0 per sub-batch), or per sub-batch
         self.permute_ahead = True if permute_ahead is None else bool(permute_ahead)
         C = max((M + nat.PC_CHUNK - 1) // nat.PC_CHUNK, 1)
-        self.prefetch = bool(prefetch) and self.permute_ahead and seeds is None
+        self.prefetch = bool(prefetch) and self.permute_ahead
         self.prefetch_after = max(0, int(prefetch_after))
         self.perms = [torch.empty((self.R, max(M, 1)), dtype=torch.int32, device=dev)
                       for _ in range(2 if self.prefetch else 1)]
@@ -458,9 +462,13 @@ class Pipeline(object):
                 # run before this one, which read it, has ended)
                 sp.wait_event(e1)
                 with torch.cuda.stream(sp):
-                    eng.device_permutations(self.seed, self.R,
-                                            out=self.perms[1 - self.cur],
-                                            workspace=self.perm_ws)
+                    if self.d_seeds is None:
+                        eng.device_permutations(self.seed, self.R,
+                                                out=self.perms[1 - self.cur],
+                                                workspace=self.perm_ws)
+                    else:
+                        eng.randomstate_permutations(self.d_seeds,
+                                                     out=self.perms[1 - self.cur])
         if self.prefetch:
             self.cur = 1 - self.cur
             self.prefetched = True

@@ -257,19 +257,25 @@ def test_lattice_sweep_rejects_bad_sizes(gpu):
     assert lib.pc_sweep_lattice(20000, 0, *args) == -1
 
 
-@pytest.mark.parametrize('ahead', [True, False])
-def test_pipeline_bit_exact_mode_matches_oracle(gpu, ahead):
+@pytest.mark.parametrize('ahead,prefetch', [(True, False), (False, False),
+                                            (True, True)])
+def test_pipeline_bit_exact_mode_matches_oracle(gpu, ahead, prefetch):
     """The pipeline with device RandomState permutations (seeds = run
     index, the L = 16 configuration) gives the oracle's exact sums over
-    numpy's permutations, drawn all before the sweeps or per sub-batch."""
+    numpy's permutations, drawn all before the sweeps, per sub-batch, or
+    ahead for the next run() (three run() calls, the later ones on
+    prefetched permutations)."""
     from pypercolate_amd._engine import Pipeline
     eng = BondEngine(_graph.square_lattice(16))
     g = oracle.percolation_graph_arrays(spanning_2d_grid(16), True)
     tr = oracle.bond_runs(g, oracle.randomstate_permutations(range(40), g['num_edges']))
     ref1, ref2 = oracle.microcanonical_sums(tr)
-    pipe = Pipeline(eng, 40, seeds=np.arange(40), sub_batch=16, permute_ahead=ahead)
-    s1, s2 = host_sums(*pipe.run())
-    assert (s1 == ref1).all() and (s2 == ref2).all()
+    pipe = Pipeline(eng, 40, seeds=np.arange(40), sub_batch=16, permute_ahead=ahead,
+                    prefetch=prefetch, prefetch_after=1)
+    assert pipe.prefetch == prefetch
+    for _ in range(3 if prefetch else 1):
+        s1, s2 = host_sums(*pipe.run())
+        assert (s1 == ref1).all() and (s2 == ref2).all()
 
 
 @pytest.mark.parametrize('M', [1, 2, 3, 12, 63, 64, 65, 129, 480, 4097, 32512,
