@@ -419,15 +419,39 @@ __device__ __forceinline__ ShufChunk shuf_chunk(uint32_t c, uint32_t M)
     return k;
 }
 
+// Run and chunk of a count or scatter block: block g * 8 * chunks + 8 c + j
+// is chunk c of run 8 g + j of the launch's rn runs, so the chunks of a run
+// go to one XCD (workgroups are dealt to the 8 XCDs in turn) in chunk order.
+// Both kernels write short pieces next to those of the neighbouring chunks
+// (a count per bucket, a segment per bucket), and the lines they share fill
+// up in that XCD's L2 instead of going out part-written.  False for the
+// padding blocks of a last group of fewer than 8 runs.
+__device__ __forceinline__ bool shuf_block(uint32_t chunks, uint32_t rn,
+                                           uint32_t &rl, uint32_t &c)
+{
+    const uint32_t per_group = 8u * chunks;
+    rl = blockIdx.x / per_group * 8u + blockIdx.x % 8u;
+    c = blockIdx.x % per_group / 8u;
+    return rl < rn;
+}
+
+__host__ __forceinline__ uint32_t shuf_blocks(uint32_t chunks, uint32_t rn)
+{
+    return (rn + 7u) / 8u * 8u * chunks;
+}
+
 // Count kernel: counts[(r * nb1 + b) * chunks + c] = elements of chunk c of
 // run r with bucket label b.
 __global__ __launch_bounds__(64 * SHUF_W) void shuffle_count_kernel(
     uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t M, uint32_t lb1,
-    uint32_t chunks, uint32_t r0, uint32_t *__restrict__ counts)
+    uint32_t chunks, uint32_t r0, uint32_t rn, uint32_t *__restrict__ counts)
 {
     extern __shared__ uint32_t cnt[];  // 2^lb1
     const uint32_t tid = threadIdx.x, lane = tid & 63u, nb1 = 1u << lb1;
-    const uint32_t c = blockIdx.x, r = r0 + blockIdx.y, run = first_run + r;
+    uint32_t rl, c;
+    if (!shuf_block(chunks, rn, rl, c))
+        return;
+    const uint32_t r = r0 + rl, run = first_run + r;
     for (uint32_t i = tid; i < nb1; i += 64u * SHUF_W)
         cnt[i] = 0u;
     __syncthreads();
@@ -510,10 +534,10 @@ __global__ __launch_bounds__(256) void shuffle_scan_kernel(
 // Scatter kernel: the elements of chunk c of run r, sorted by bucket label
 // in LDS (stable), then written bucket by bucket with consecutive threads on
 // consecutive addresses (whole cache lines, instead of one scattered 4-byte
-// store per element).
+// store per element); blocks laid out by shuf_block.
 __global__ __launch_bounds__(64 * SHUF_W) void shuffle_scatter_kernel(
     uint32_t k0, uint32_t k1, uint32_t first_run, uint32_t M, uint32_t lb1,
-    uint32_t chunks, uint32_t r0, const uint32_t *__restrict__ counts,
+    uint32_t chunks, uint32_t r0, uint32_t rn, const uint32_t *__restrict__ counts,
     int32_t *__restrict__ perm, int64_t stride)
 {
     __shared__ int32_t sval[SHUF_CHUNK];
@@ -523,7 +547,10 @@ __global__ __launch_bounds__(64 * SHUF_W) void shuffle_scatter_kernel(
     const uint32_t T = 64u * SHUF_W, tid = threadIdx.x, lane = tid & 63u,
                    wave = tid >> 6;
     const uint32_t nb1 = 1u << lb1;
-    const uint32_t c = blockIdx.x, r = r0 + blockIdx.y, run = first_run + r;
+    uint32_t rl, c;
+    if (!shuf_block(chunks, rn, rl, c))
+        return;
+    const uint32_t r = r0 + rl, run = first_run + r;
     uint32_t *base = dyn, *lstart = dyn + SHUF_W * nb1, *gbase = lstart + nb1;
     uint32_t *my = base + wave * nb1;
     for (uint32_t i = tid; i < SHUF_W * nb1; i += T)
@@ -3691,15 +3718,15 @@ extern "C" int pc_permute_philox(uint64_t seed, uint64_t first_run,
             PC_LAUNCH_CHECK("shuffle_group_kernel");
             continue;
         }
-        shuffle_count_kernel<<<dim3(chunks, rn), 64 * SHUF_W, nb1 * 4u, s>>>(
-            k0, k1, fr, num_edges, lb1, chunks, r0, counts);
+        shuffle_count_kernel<<<shuf_blocks(chunks, rn), 64 * SHUF_W, nb1 * 4u, s>>>(
+            k0, k1, fr, num_edges, lb1, chunks, r0, rn, counts);
         PC_LAUNCH_CHECK("shuffle_count_kernel");
         shuffle_scan_kernel<<<rn, 256, nb1 * 4u, s>>>(num_edges, lb1, chunks, r0,
                                                       counts, starts);
         PC_LAUNCH_CHECK("shuffle_scan_kernel");
-        shuffle_scatter_kernel<<<dim3(chunks, rn), 64 * SHUF_W,
+        shuffle_scatter_kernel<<<shuf_blocks(chunks, rn), 64 * SHUF_W,
                                  (SHUF_W + 2u) * nb1 * 4u, s>>>(
-            k0, k1, fr, num_edges, lb1, chunks, r0, counts, d_perm, perm_stride);
+            k0, k1, fr, num_edges, lb1, chunks, r0, rn, counts, d_perm, perm_stride);
         PC_LAUNCH_CHECK("shuffle_scatter_kernel");
         shuffle_group_kernel<4, false><<<dim3(nb1, rn), 256,
                                          shuf_group_lds_bytes(4, SHUF_GROUP_CAP), s>>>(
