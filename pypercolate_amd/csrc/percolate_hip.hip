@@ -1056,22 +1056,26 @@ __global__ __launch_bounds__(64) void randomstate_targets_kernel(
     }
 }
 
-// smallest of the list entries ent[b..e) above thr, or 0xFFFF (entries are
-// steps, 1 <= s < n <= 32768); four entries are read per round
-__device__ __forceinline__ uint32_t rs_list_min_above(const uint16_t *ent,
-                                                      uint32_t b, uint32_t e,
-                                                      uint32_t thr)
+// Smallest of the list entries ent[b..e) above thr, or 0xFFFF (entries are
+// steps, 1 <= s < n <= 32768), and whether an entry below thr exists; four
+// entries are read per round.
+__device__ __forceinline__ uint32_t rs_list_scan(const uint16_t *ent, uint32_t b,
+                                                 uint32_t e, uint32_t thr, bool &below)
 {
     uint32_t best = 0xFFFFu;
+    bool lo = false;
     for (uint32_t a = b; a < e; a += 4u) {
         uint32_t x[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4u; ++u)
             x[u] = ent[min(a + u, e - 1u)];
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; ++u)
+        for (uint32_t u = 0; u < 4u; ++u) {
             best = x[u] > thr ? min(best, x[u]) : best;
+            lo |= x[u] < thr;
+        }
     }
+    below = lo;
     return best;
 }
 
@@ -1079,8 +1083,9 @@ __device__ __forceinline__ uint32_t rs_list_min_above(const uint16_t *ent,
 // edges, in place over the targets.  LDS holds 16-bit list cursors of the
 // positions in pairs and the lists of steps per target position (a counting
 // sort: counts, exclusive scan, placement; unordered within a list); every
-// thread owns indices j = tid + q * RS_APPLY_T and scans the lists of k_j
-// and of j for the smallest step above j.  The list area is then reused for
+// thread owns indices j = tid + q * RS_APPLY_T and scans the list of k_j
+// once, for the smallest step above j and for whether j is the list's
+// smallest step (which then knows F(k_j)).  The list area is then reused for
 // V, which pointer jumping (V(p) <- V(V(p)) until nothing changes, over
 // position pairs) takes to the chain roots.  The loads of independent items
 // are issued together before any is waited for.  Dynamic LDS:
@@ -1180,44 +1185,45 @@ __global__ __launch_bounds__(RS_APPLY_T) void randomstate_apply_kernel(
         }
     }
     __syncthreads();
-    // per index, packed: low half the result, or LOOKUP | s for "V(s)";
-    // high half F(j), or j without one (all values are below 2^15).  The
-    // list bounds of a group of indices are read before the lists.
+    // Each step j scans the list of its target p = k_j once: the smallest
+    // step above j there (its successor), and whether j is the list's
+    // smallest step, in which case j knows F(p): j itself when j > p, else
+    // (j = p) its successor.  Per index, packed: low half the result, or
+    // LOOKUP | s for "V(s)"; bit 31 set for a list's smallest step, with
+    // F(p) in bits 16..30 (all values are below 2^15).  The list bounds of a
+    // group of indices are read before the lists.
     auto half = [&](uint32_t x, uint32_t p) { return (x >> ((p & 1u) * 16u)) & 0xFFFFu; };
     uint32_t rf[PER];
     const uint32_t nr = opaque(n), tr = opaque_v(tid);
 #pragma unroll
     for (uint32_t g = 0; g < PER; g += GROUP) {
-        uint32_t se[GROUP], sb[GROUP], fe[GROUP], fb[GROUP];
+        uint32_t se[GROUP], sb[GROUP];
 #pragma unroll
         for (uint32_t u = 0; u < GROUP; ++u) {
-            const uint32_t j = min(tr + (g + u) * T, nr - 1u), p = k[g + u];
+            const uint32_t p = k[g + u];
             se[u] = cur[p >> 1];
             sb[u] = cur[p ? (p - 1u) >> 1 : 0u];     // unused for p = 0
-            fe[u] = cur[j >> 1];
-            fb[u] = cur[j ? (j - 1u) >> 1 : 0u];
         }
 #pragma unroll
         for (uint32_t u = 0; u < GROUP; ++u) {
             const uint32_t q = g + u, j = tr + q * T;
-            uint32_t r = LOOKUP, f = j;
-            if (j < nr) {
+            uint32_t r = LOOKUP, w = 0u;
+            if (j < nr && j) {
                 const uint32_t p = k[q];
-                if (j) {
-                    const uint32_t b = p ? half(sb[u], p - 1u) : 0u, e = half(se[u], p);
-                    const uint32_t s = rs_list_min_above(ent, b, e, j);
-                    r = s == 0xFFFFu ? p : (LOOKUP | s);
-                }
-                const uint32_t b = j ? half(fb[u], j - 1u) : 0u, e = half(fe[u], j);
-                const uint32_t s = rs_list_min_above(ent, b, e, j);
-                f = s == 0xFFFFu ? j : s;
+                const uint32_t b = p ? half(sb[u], p - 1u) : 0u, e = half(se[u], p);
+                bool below;
+                const uint32_t s = rs_list_scan(ent, b, e, j, below);
+                r = s == 0xFFFFu ? p : (LOOKUP | s);
+                if (!below)
+                    w = 0x80000000u | ((j > p ? j : (s == 0xFFFFu ? p : s)) << 16);
             }
-            rf[q] = r | (f << 16);
+            rf[q] = r | w;
         }
     }
     __syncthreads();
-    // the results move to the cursor area, V (initialised with F) to the
-    // list area
+    // the results move to the cursor area; V, in the list area, starts as
+    // every position itself (the positions no step targets keep that), then
+    // each list's smallest step writes F of its target
     uint16_t *res = (uint16_t *)cur;
     {
         uint16_t *Vw = ent;
@@ -1227,9 +1233,14 @@ __global__ __launch_bounds__(RS_APPLY_T) void randomstate_apply_kernel(
             const uint32_t j = tv + q * T;
             if (j < nv) {
                 res[j] = (uint16_t)rf[q];
-                Vw[j] = (uint16_t)(rf[q] >> 16);
+                Vw[j] = (uint16_t)j;
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < PER; ++q)
+            if (rf[q] >> 31)
+                Vw[k[q]] = (uint16_t)((rf[q] >> 16) & 0x7FFFu);
     }
     __syncthreads();
     // pointer jumping over the pairs; a concurrent update only moves a

@@ -88,6 +88,16 @@ def apply_swaps(k, n, first=True):
     for p in range(n):
         later = [s for s in lists[p] if s > p]
         F.append(pick(later) if later else p)
+    # the kernel's way: the smallest step j of each list knows F of its
+    # target p, j itself when j > p, else (j = p) its successor
+    G = list(range(n))
+    for j in range(1, n):
+        p = k[j]
+        if min(lists[p]) == j:
+            later = [s for s in lists[p] if s > j]
+            G[p] = j if j > p else (min(later) if later else p)
+    if first:
+        assert G == F
     V = F[:]
     while True:
         nxt = [V[V[p]] for p in range(n)]
