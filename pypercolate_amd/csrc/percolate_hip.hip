@@ -1528,8 +1528,10 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 // barrier per batch, the roots handed over through LDS.  A find that runs
 // beside the merges of batch b may stop at a root linked meanwhile, which is
 // still an ancestor: the resolver's find continues from it, usually with a
-// single read.  With chunk_out, the finder also folds the records of batch
-// b - 1 into the running state, so the reductions stay off the resolver.
+// single read.  With chunk_out and the state in LDS, the finder folds the
+// records of batch b - 1 into the running state, off the resolver's chain;
+// with the state in HBM the finder's dependent reads set the pace, so the
+// resolver folds its own records in the time it would wait.
 // Edge sources of the sweep: edge e -> its endpoints (u, v).
 // EdgeList reads the graph's edge array.  LatticeEdges computes them for the
 // square lattice of side L in the order square_lattice builds (rows r =
@@ -1598,6 +1600,9 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     constexpr bool WIDE = std::is_same<St, GlobalState>::value;
     using MT = typename Wide<WIDE>::T;
     constexpr uint32_t THREADS = SPLIT ? 128u : 64u;
+    // with the state in HBM the finder is the slower wave: the resolver
+    // folds its own records into the chunk states
+    constexpr bool RES_FOLD = SPLIT && WIDE;
     const uint32_t lane = threadIdx.x % 64u;
     // SPLIT: wave 0 resolves batch b while wave 1 finds the roots of batch
     // b + 1 (a find that runs beside the merges of batch b may stop at a
@@ -1689,7 +1694,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 
     for (uint32_t base = 0; base < M; base += 64u) {
         if (SPLIT && !resolver) {
-            if (CHUNKS) {
+            if (CHUNKS && !RES_FOLD) {
                 // state at the start of this batch: fold the records the
                 // resolver finished in the previous iteration
                 if (base > 0u)
@@ -1705,7 +1710,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         }
         const uint32_t n = base + lane;
         const bool active = n < M;
-        if (!SPLIT && CHUNKS && lane < PC_STATE_WORDS)
+        if ((!SPLIT || RES_FOLD) && CHUNKS && lane < PC_STATE_WORDS)
             chunk_out[(size_t)(base / 64u) * PC_STATE_WORDS + lane] =
                 state_word(lane, cs_mx, cs_merges, cs_s2, cs_s3, cs_s4);
         uint32_t ru, rv, cu, cv;
@@ -1875,7 +1880,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         }
         if (active)
             rec_out[n] = rec;
-        if (SPLIT && CHUNKS)
+        if (SPLIT && CHUNKS && !RES_FOLD)
             rec_slots[((base / 64u) & 1u) * 64u + lane] = rec;
         else if (CHUNKS)
             fold_batch(rec);
