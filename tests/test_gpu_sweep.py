@@ -135,10 +135,11 @@ def test_device_permutation_matches_oracle(gpu, M):
     4096) and the 8-wave group kernel (up to 32768: the L = 128 lattice),
     one sub-bucket or several, then several buckets through count, scan and
     scatter (from 32769 edges; several chunks of those passes at the
-    L = 256 and L = 1024 lattices)."""
+    L = 256 and L = 1024 lattices; the count and scatter blocks of 11 runs
+    span two groups of eight runs, the second padded)."""
     eng = BondEngine(_graph.GraphArrays(M + 1, np.stack(
         [np.arange(M), np.arange(1, M + 1)], axis=1)))
-    runs = 3 if M < 1000000 else 2
+    runs = 2 if M >= 1000000 else (11 if M > 32768 else 3)
     d = eng.device_permutations(seed=0x1234567890AB, num_runs=runs, first_run=5)
     got = d.cpu().numpy()
     want = oracle.bucket_permutations(0x1234567890AB, 5, runs, M)
