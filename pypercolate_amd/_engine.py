@@ -320,8 +320,9 @@ class Pipeline(object):
     idle), chunk states and sums on a fourth (each waiting for its sweep,
     accumulating into one pair of sum buffers).  Four streams in all, one
     per hardware queue of the process (HIP's default of four): streams that
-    shared a queue would serialise.  HBM-state graphs get their chunk states from the sweep itself
-    (BondEngine.fuse_chunk_states).  All buffers are allocated up front.
+    shared a queue would serialise.  HBM-state graphs get their chunk
+    states from the sweep itself (BondEngine.fuse_chunk_states).  All
+    buffers are allocated up front.
 
     Permutations: Philox-keyed (pc_permute_philox, seed) by default, or
     with seeds= (R integers) numpy RandomState(seeds[r]).permutation(M),
@@ -334,10 +335,9 @@ class Pipeline(object):
     the next run() starts with its sweeps (RandomState permutations too:
     10.4 ms per step of 10^4 runs at L = 128 with prefetch_after = 2,
     against 11.1 without; started earlier, they stretch the sweeps by about
-    as much as they hide).  The
-    first run() draws its own.  Every run() still draws one batch of
-    permutations, the same ones (the permutations depend only on the seeds,
-    or the seed and the run index)."""
+    as much as they hide).  The first run() draws its own.  Every run()
+    still draws one batch of permutations, the same ones (the permutations
+    depend only on the seeds, or the seed and the run index)."""
 
     def __init__(self, engine, num_runs, seed=0, sub_batch=None, seeds=None,
                  permute_ahead=None, prefetch=False, prefetch_after=1):
