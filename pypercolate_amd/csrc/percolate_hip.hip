@@ -1079,17 +1079,6 @@ __device__ __forceinline__ uint32_t rs_list_scan(const uint16_t *ent, uint32_t b
     return best;
 }
 
-// Phase two, one workgroup of RS_APPLY_T threads per run of n <= RS_APPLY_MAX
-// edges, in place over the targets.  LDS holds 16-bit list cursors of the
-// positions in pairs and the lists of steps per target position (a counting
-// sort: counts, exclusive scan, placement; unordered within a list); every
-// thread owns indices j = tid + q * RS_APPLY_T and scans the list of k_j
-// once, for the smallest step above j and for whether j is the list's
-// smallest step (which then knows F(k_j)).  The list area is then reused for
-// V, which pointer jumping (V(p) <- V(V(p)) until nothing changes, over
-// position pairs) takes to the chain roots.  The loads of independent items
-// are issued together before any is waited for.  Dynamic LDS:
-// rs_apply_lds_bytes(n).
 // A copy of x the compiler cannot relate to x: predicates computed from it in
 // one phase are not kept alive (as 32 lane masks) for a later phase.
 __device__ __forceinline__ uint32_t opaque(uint32_t x)
@@ -1103,6 +1092,17 @@ __device__ __forceinline__ uint32_t opaque_v(uint32_t x)
     return x;
 }
 
+// Phase two, one workgroup of RS_APPLY_T threads per run of n <= RS_APPLY_MAX
+// edges, in place over the targets.  LDS holds 16-bit list cursors of the
+// positions in pairs and the lists of steps per target position (a counting
+// sort: counts, exclusive scan, placement; unordered within a list); every
+// thread owns indices j = tid + q * RS_APPLY_T and scans the list of k_j
+// once, for the smallest step above j and for whether j is the list's
+// smallest step (which then knows F(k_j)).  The list area is then reused for
+// V, which pointer jumping (V(p) <- V(V(p)) until nothing changes, over
+// position pairs) takes to the chain roots.  The loads of independent items
+// are issued together before any is waited for.  Dynamic LDS:
+// rs_apply_lds_bytes(n).
 __global__ __launch_bounds__(RS_APPLY_T) void randomstate_apply_kernel(
     uint32_t n, int32_t *__restrict__ perm, int64_t stride)
 {
