@@ -8,7 +8,9 @@ runs for every candidate
 and keeps the fastest.  The sample is one residency's worth of runs (the
 HIP occupancy calculator's workgroups per CU times the number of CUs, at
 most about 2^28 bonds of buffers), so the timing sees the concurrency of a
-full launch.  None of the knobs
+full launch, or the caller's own number of runs when that is smaller
+(``max_runs``: a launch of a few runs per CU is bound by latency rather
+than by the memory system, and may prefer other knobs).  None of the knobs
 changes the results; ``tests/test_gpu_batch.py`` checks that the tuned
 sweep is bit-identical to the default one.
 
@@ -80,15 +82,19 @@ def _time_sweep(engine, perm, records, span, reps=3):
 
 
 def tune_sweep(engine, sample_runs=None, rounds=ROUND_CANDIDATES, modes=None,
-               use_cache=True, seed=0):
+               use_cache=True, seed=0, max_runs=None):
     """Pick (merge_rounds, state_mode) for ``engine`` (a BondEngine), set
     them on it and return a dict with the choice and the timings.
     state_mode candidates are PC_STATE_LDS / PC_STATE_HBM, each with one
-    and with two wavefronts per run (PC_SWEEP_*)."""
+    and with two wavefronts per run (PC_SWEEP_*).  ``max_runs``: the most
+    runs the caller sweeps in one launch; the sample is cut to it, and the
+    cache keeps one choice per graph and cap."""
     torch = engine.torch
     device = torch.cuda.get_device_properties(engine.device)
     key = '{}|{}|{}'.format(device.gcnArchName, device.multi_processor_count,
                             graph_key(engine.graph))
+    if max_runs:
+        key += '|runs<={}'.format(int(max_runs))
     cache = _load_cache() if use_cache else {}
     if key in cache:
         choice = cache[key]
@@ -109,6 +115,8 @@ def tune_sweep(engine, sample_runs=None, rounds=ROUND_CANDIDATES, modes=None,
         # one residency of runs, capped at about 2^28 bonds of buffers
         cap = max(64, (1 << 28) // max(engine.M, 1))
         runs = sample_runs or max(1, min(occ * cus, 8192, cap))
+        if max_runs:
+            runs = min(runs, int(max_runs))
         perm = engine.device_permutations(seed, runs)
         records = torch.empty((runs, max(engine.M, 1)), dtype=engine.rec_torch,
                               device=engine.device)

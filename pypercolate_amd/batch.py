@@ -43,7 +43,7 @@ class _LengthPlan(object):
         eng = self.engine = BondEngine(_graph.square_lattice(length))
         torch = eng.torch
         if tune:
-            self.tuning = tune_sweep(eng)
+            self.tuning = tune_sweep(eng, max_runs=runs)
         else:
             self.tuning = None
         dev = eng.device
