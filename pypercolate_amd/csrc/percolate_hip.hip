@@ -1358,6 +1358,42 @@ __device__ __forceinline__ void unpack_record(uint64_t r, uint32_t &a, uint32_t 
     b = (uint32_t)(r >> 32);
 }
 
+// The HBM-state loop of find2, from codes cx, cy already read (rx, ry:
+// whether they are root codes): path halving until both are roots.
+template <class St>
+__device__ __forceinline__ void follow2(const St &st, uint32_t &x, uint32_t &cx, bool rx,
+                                        uint32_t &y, uint32_t &cy, bool ry)
+{
+    while (!(rx && ry)) {
+        const uint32_t gx = rx ? cx : st.get(cx);
+        const uint32_t gy = ry ? cy : st.get(cy);
+        if (!rx) {
+            if (st.is_root(gx)) {
+                x = cx;
+                cx = gx;
+                rx = true;
+            } else {
+                st.set(x, gx);
+                x = gx;
+                cx = st.get(x);
+                rx = st.is_root(cx);
+            }
+        }
+        if (!ry) {
+            if (st.is_root(gy)) {
+                y = cy;
+                cy = gy;
+                ry = true;
+            } else {
+                st.set(y, gy);
+                y = gy;
+                cy = st.get(y);
+                ry = st.is_root(cy);
+            }
+        }
+    }
+}
+
 // Two interleaved finds with path halving.  Concurrent halving by other
 // lanes of the wave only ever replaces a parent by an ancestor, so every
 // lane still reaches its true root.  With the state in LDS each pass is
@@ -1393,34 +1429,7 @@ __device__ __forceinline__ void find2(const St &st, uint32_t &x, uint32_t &cx,
         }
         return;
     }
-    while (!(rx && ry)) {
-        const uint32_t gx = rx ? cx : st.get(cx);
-        const uint32_t gy = ry ? cy : st.get(cy);
-        if (!rx) {
-            if (st.is_root(gx)) {
-                x = cx;
-                cx = gx;
-                rx = true;
-            } else {
-                st.set(x, gx);
-                x = gx;
-                cx = st.get(x);
-                rx = st.is_root(cx);
-            }
-        }
-        if (!ry) {
-            if (st.is_root(gy)) {
-                y = cy;
-                cy = gy;
-                ry = true;
-            } else {
-                st.set(y, gy);
-                y = gy;
-                cy = st.get(y);
-                ry = st.is_root(cy);
-            }
-        }
-    }
+    follow2(st, x, cx, rx, y, cy, ry);
 }
 
 // ---------------------------------------------------------------------------
@@ -1492,6 +1501,18 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
     return ((root & 0xFFFFFFu) * 0x9E3779u) >> (32 - CLAIM_BITS);
 }
 
+// Modified-root stamps of an HBM-state sweep: DIRTY_SLOTS words in LDS,
+// indexed by the same kind of hash; a slot holds 1 + the last batch in
+// which the resolver changed a root hashing there (0: never).  Collisions
+// only make a root look changed.
+#define DIRTY_BITS 11
+#define DIRTY_SLOTS (1u << DIRTY_BITS)
+
+__device__ __forceinline__ uint32_t dirty_slot(uint32_t root)
+{
+    return ((root & 0xFFFFFFu) * 0x9E3779u) >> (32 - DIRTY_BITS);
+}
+
 // One run on one wavefront.  Bonds are taken 64 at a time (lane = step):
 //  1. every lane finds the roots of its bond's endpoints in parallel (path
 //     halving); lanes whose roots differ are candidate merges;
@@ -1531,7 +1552,12 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t root)
 // single read.  With chunk_out and the state in LDS, the finder folds the
 // records of batch b - 1 into the running state, off the resolver's chain;
 // with the state in HBM the finder's dependent reads set the pace, so the
-// resolver folds its own records in the time it would wait.
+// resolver folds its own records in the time it would wait.  With the state
+// in HBM the finder also hands over the code (size and sides) of each root
+// it found, and the resolver stamps every root it changes with the batch in
+// an LDS table (hashed, so a collision only costs a read): a handed root
+// whose stamp is older than the batch the finder ran beside has not changed
+// since the finder read it, so the resolver takes it without a read.
 // Edge sources of the sweep: edge e -> its endpoints (u, v).
 // EdgeList reads the graph's edge array.  LatticeEdges computes them for the
 // square lattice of side L in the order square_lattice builds (rows r =
@@ -1586,6 +1612,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           bool span_first, uint32_t rounds,
                                           uint32_t *__restrict__ claims,
                                           uint32_t *__restrict__ handoff,
+                                          uint32_t *__restrict__ dirty,
                                           const int32_t *__restrict__ perm,
                                           RecT *__restrict__ rec_out,
                                           uint32_t *__restrict__ span_out,
@@ -1603,6 +1630,12 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     // with the state in HBM the finder is the slower wave: the resolver
     // folds its own records into the chunk states
     constexpr bool RES_FOLD = SPLIT && WIDE;
+    // HBM state with a finder: the finder hands over the codes of the roots
+    // it found as well (after the two root slots), and the resolver reads a
+    // root afresh only if, in the batch before (beside which the finder
+    // read it), it changed a root hashing to the same stamp slot
+    constexpr bool HANDED = SPLIT && WIDE;
+    uint32_t *const codes_out = handoff + 256u;
     const uint32_t lane = threadIdx.x % 64u;
     // SPLIT: wave 0 resolves batch b while wave 1 finds the roots of batch
     // b + 1 (a find that runs beside the merges of batch b may stop at a
@@ -1615,6 +1648,9 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         st.set(i, st.root(1u, sides ? (uint32_t)sides[i] : 0u));
     for (uint32_t i = threadIdx.x; i < CLAIM_SLOTS; i += THREADS)
         claims[i] = 0u;
+    if (HANDED)
+        for (uint32_t i = threadIdx.x; i < DIRTY_SLOTS; i += THREADS)
+            dirty[i] = 0u;
     __syncthreads();
 
     uint32_t span_at = PC_NO_SPAN;
@@ -1667,11 +1703,17 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         uint32_t x = st.is_root(hu) ? (uint32_t)uv.x : hu;
         uint32_t y = st.is_root(hv) ? (uint32_t)uv.y : hv;
         uint32_t cx, cy;
+        if (HANDED)
+            cx = cy = st.root(1u, 0u);   // a root code on idle lanes
         if (b0 + lane < M)
             find2(st, x, cx, y, cy);
-        uint32_t *slot = handoff + ((b0 / 64u) & 1u) * 128u;
-        slot[lane] = x;
-        slot[64u + lane] = y;
+        const uint32_t si = ((b0 / 64u) & 1u) * 128u;
+        handoff[si + lane] = x;
+        handoff[si + 64u + lane] = y;
+        if (HANDED) {
+            codes_out[si + lane] = cx;
+            codes_out[si + 64u + lane] = cy;
+        }
     };
     if (SPLIT) {
         if (!resolver)
@@ -1716,10 +1758,23 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         uint32_t ru, rv, cu, cv;
         if (SPLIT) {
             // the finder's roots, followed past the previous batch's merges
-            const uint32_t *slot = handoff + ((base / 64u) & 1u) * 128u;
-            ru = slot[lane];
-            rv = slot[64u + lane];
-            find2(st, ru, cu, rv, cv);
+            const uint32_t si = ((base / 64u) & 1u) * 128u;
+            ru = handoff[si + lane];
+            rv = handoff[si + 64u + lane];
+            if (HANDED) {
+                // a root unchanged since the finder read it is still a root
+                // with that code: no read
+                const uint32_t b = base / 64u;
+                cu = codes_out[si + lane];
+                cv = codes_out[si + 64u + lane];
+                if (dirty[dirty_slot(ru)] >= b)
+                    cu = st.get(ru);
+                if (dirty[dirty_slot(rv)] >= b)
+                    cv = st.get(rv);
+                follow2(st, ru, cu, st.is_root(cu), rv, cv, st.is_root(cv));
+            } else {
+                find2(st, ru, cu, rv, cv);
+            }
         } else {
             const int2 uv = uv_next;
             uv_next = (n + 64u < M) ? edges(e_ahead) : make_int2(0, 0);
@@ -1799,11 +1854,16 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 if (in_star) {
                     // (size on u's side, size on v's side), as every merge
                     st.set(x, hub);
+                    if (HANDED)
+                        dirty[dirty_slot(x)] = base / 64u + 1u;
                     const uint32_t sh_now = s_hub + incl - sx;
                     rec = pack_record<RecT>(at_u ? sh_now : sx, at_u ? sx : sh_now);
                 }
-                if (lane == first)
+                if (lane == first) {
                     st.set(hub, st.root(s_hub + total, f_all == 3u ? 0u : f_all));
+                    if (HANDED)
+                        dirty[dirty_slot(hub)] = base / 64u + 1u;
+                }
                 if (span_first)
                     span_at = min(span_at, base + first + 1u);
                 if (f_all == 3u) {
@@ -1823,6 +1883,10 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                 uint32_t f = fa | fb;
                 st.set(small, big);
                 st.set(big, st.root(sa + sb, f == 3u ? 0u : f));
+                if (HANDED) {
+                    dirty[dirty_slot(small)] = base / 64u + 1u;
+                    dirty[dirty_slot(big)] = base / 64u + 1u;
+                }
                 rec = pack_record<RecT>(sa, sb);
                 spans_here = span_first || f == 3u;
             }
@@ -1875,6 +1939,10 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
             if (lane == 0) {
                 st.set(small, big);
                 st.set(big, st.root(sa + sb, f));
+                if (HANDED) {
+                    dirty[dirty_slot(small)] = base / 64u + 1u;
+                    dirty[dirty_slot(big)] = base / 64u + 1u;
+                }
             }
             rec = write_lane(pack_record<RecT>(sa, sb), i, rec);
         }
@@ -1897,7 +1965,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 
 // Words of LDS before the 16-bit codes: the claim table, then (SPLIT) the
 // handoff: two slots of 128 roots, then two slots of 64 records (up to 64
-// bits each).
+// bits each; with the state in HBM, two slots of the roots' 128 codes).
 #define SWEEP_HANDOFF_WORDS 512u
 template <bool SPLIT>
 constexpr uint32_t sweep_lds_words() { return CLAIM_SLOTS + (SPLIT ? SWEEP_HANDOFF_WORDS : 0u); }
@@ -1919,7 +1987,7 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
     const LdsState st{(uint16_t *)(lds_words + sweep_lds_words<SPLIT>()), 1u << shift, shift};
     sweep_run<LdsState, RecT, CHUNKS, SPLIT, ES>(
         st, N, M, edges, sides, span_first != 0, rounds, lds_words,
-        lds_words + CLAIM_SLOTS, perm + (int64_t)r * perm_stride,
+        lds_words + CLAIM_SLOTS, nullptr, perm + (int64_t)r * perm_stride,
         records + (int64_t)r * rec_stride, span_step + r,
         CHUNKS ? chunk_state + r * cstride : nullptr);
 }
@@ -1935,12 +2003,13 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
 {
     __shared__ uint32_t claims[CLAIM_SLOTS];
     __shared__ uint32_t handoff[SPLIT ? SWEEP_HANDOFF_WORDS : 1u];
+    __shared__ uint32_t dirty[SPLIT ? DIRTY_SLOTS : 1u];
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
         sweep_run<GlobalState, RecT, CHUNKS, SPLIT, ES>(
             st, N, M, edges, sides, span_first != 0, rounds, claims, handoff,
-            perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
+            dirty, perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
             span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
         __syncthreads();
     }
@@ -1980,7 +2049,9 @@ __global__ __launch_bounds__(128) void sweep_multi_kernel(
     extern __shared__ uint32_t lds_words[];
     __shared__ uint32_t item;
     uint32_t *claims = lds_words, *handoff = lds_words + CLAIM_SLOTS;
+    // after the handoff: an LDS graph's codes, or an HBM graph's root stamps
     uint16_t *codes = (uint16_t *)(lds_words + sweep_lds_words<true>());
+    uint32_t *dirty = lds_words + sweep_lds_words<true>();
     for (;;) {
         if (threadIdx.x == 0u)
             item = atomicAdd(head, 1u);
@@ -2002,19 +2073,20 @@ __global__ __launch_bounds__(128) void sweep_multi_kernel(
                 const LdsState st{codes, 1u << G.shift, G.shift};
                 sweep_run<LdsState, uint32_t, true, true>(
                     st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
-                    claims, handoff, perm, (uint32_t *)G.records + (int64_t)r * G.rec_stride,
+                    claims, handoff, nullptr, perm,
+                    (uint32_t *)G.records + (int64_t)r * G.rec_stride,
                     G.span_step + r, cs);
             } else {
                 const GlobalState st{workspace + (size_t)blockIdx.x * slot_words};
                 if (G.kind == MULTI_HBM32)
                     sweep_run<GlobalState, uint32_t, true, true>(
                         st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
-                        claims, handoff, perm,
+                        claims, handoff, dirty, perm,
                         (uint32_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
                 else
                     sweep_run<GlobalState, uint64_t, true, true>(
                         st, G.N, G.M, edges, G.sides, G.span_first != 0u, G.rounds,
-                        claims, handoff, perm,
+                        claims, handoff, dirty, perm,
                         (uint64_t *)G.records + (int64_t)r * G.rec_stride, G.span_step + r, cs);
             }
         };
@@ -4535,6 +4607,8 @@ static int multi_plan(uint32_t num_graphs, const pc_sweep_graph *graphs,
         } else {
             G.kind = in.num_nodes <= kU32RecordMaxNodes ? MULTI_HBM32 : MULTI_HBM64;
             plan.slot_words = std::max<uint64_t>(plan.slot_words, in.num_nodes);
+            plan.lds = std::max(plan.lds, (sweep_lds_words<true>() + DIRTY_SLOTS) *
+                                              sizeof(uint32_t));
         }
         plan.total += in.num_runs;
     }
