@@ -1605,7 +1605,7 @@ struct LatticeEdges {
     }
 };
 
-template <class St, typename RecT, bool CHUNKS, bool SPLIT, class ES>
+template <class St, typename RecT, bool CHUNKS, bool SPLIT, class ES, bool FOLDER = false>
 __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                                           const ES &edges,
                                           const uint8_t *__restrict__ sides,
@@ -1626,10 +1626,13 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     // HBM-state graphs (N > 16384) need 128-bit S3 and S4
     constexpr bool WIDE = std::is_same<St, GlobalState>::value;
     using MT = typename Wide<WIDE>::T;
-    constexpr uint32_t THREADS = SPLIT ? 128u : 64u;
-    // with the state in HBM the finder is the slower wave: the resolver
-    // folds its own records into the chunk states
-    constexpr bool RES_FOLD = SPLIT && WIDE;
+    constexpr uint32_t THREADS = SPLIT ? (FOLDER ? 192u : 128u) : 64u;
+    // who folds the records into the chunk states: with the state in LDS
+    // the finder (the resolver sets the pace); with the state in HBM a third
+    // wave (FOLDER), or else the resolver
+    static_assert(!FOLDER || (SPLIT && CHUNKS && std::is_same<St, GlobalState>::value),
+                  "a folder wave only for chunk states with the state in HBM");
+    constexpr bool RES_FOLD = SPLIT && WIDE && !FOLDER;
     // HBM state with a finder: the finder hands over the codes of the roots
     // it found as well (after the two root slots), and the resolver reads a
     // root afresh only if, in the batch before (beside which the finder
@@ -1642,6 +1645,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     // root linked meanwhile; that is still an ancestor, and the resolver's
     // find continues from it)
     const bool resolver = !SPLIT || threadIdx.x < 64u;
+    const bool folder = FOLDER && threadIdx.x >= 128u;
     uint64_t cs_mx = 1, cs_merges = 0, cs_s2 = N;
     MT cs_s3 = N, cs_s4 = N;
     for (uint32_t i = threadIdx.x; i < N; i += THREADS)
@@ -1683,7 +1687,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
     int2 f_uv1 = make_int2(0, 0), f_uv2 = make_int2(0, 0);
     int32_t f_e3 = 0;
     uint32_t f_hu = 0xFFFFFFFFu, f_hv = 0xFFFFFFFFu;
-    if (SPLIT && !resolver) {
+    if (SPLIT && !resolver && !folder) {
         if (lane < M)
             f_uv1 = edges(perm[lane]);
         if (64u + lane < M)
@@ -1716,15 +1720,16 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
         }
     };
     if (SPLIT) {
-        if (!resolver)
+        if (!resolver && !folder)
             find_batch(0u);
         __syncthreads();
     }
 
     // SPLIT with chunk states: the resolver hands each batch's records to
-    // the finder (two slots of 64 records after the root slots), which folds
-    // them into the running state one batch later, off the resolver's chain
-    RecT *rec_slots = (RecT *)(handoff + 256u);
+    // the wave that folds them (two slots of 64 records after the root
+    // slots, after the roots' codes with the state in HBM), which folds them
+    // into the running state one batch later, off the resolver's chain
+    RecT *rec_slots = (RecT *)(handoff + (HANDED ? 512u : 256u));
     auto fold_batch = [&](RecT rec) {
         const Deltas<WIDE> d(rec);
         cs_mx = umax64(cs_mx, (uint64_t)read_lane(wave_scan_max(d.dmax), 63u));
@@ -1736,7 +1741,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
 
     for (uint32_t base = 0; base < M; base += 64u) {
         if (SPLIT && !resolver) {
-            if (CHUNKS && !RES_FOLD) {
+            if (CHUNKS && !RES_FOLD && (folder || !FOLDER)) {
                 // state at the start of this batch: fold the records the
                 // resolver finished in the previous iteration
                 if (base > 0u)
@@ -1745,7 +1750,7 @@ __device__ __forceinline__ void sweep_run(const St &st, uint32_t N, uint32_t M,
                     chunk_out[(size_t)(base / 64u) * PC_STATE_WORDS + lane] =
                         state_word(lane, cs_mx, cs_merges, cs_s2, cs_s3, cs_s4);
             }
-            if (base + 64u < M)
+            if (!folder && base + 64u < M)
                 find_batch(base + 64u);
             __syncthreads();
             continue;
@@ -1992,8 +1997,8 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_lds_kernel(
         CHUNKS ? chunk_state + r * cstride : nullptr);
 }
 
-template <typename RecT, bool CHUNKS, bool SPLIT, class ES>
-__global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
+template <typename RecT, bool CHUNKS, bool SPLIT, class ES, bool FOLDER = false>
+__global__ __launch_bounds__(SPLIT ? (FOLDER ? 192 : 128) : 64) void sweep_global_kernel(
     uint32_t N, uint32_t M, uint32_t R, ES edges,
     const uint8_t *__restrict__ sides, int span_first,
     const int32_t *__restrict__ perm, int64_t perm_stride,
@@ -2001,13 +2006,14 @@ __global__ __launch_bounds__(SPLIT ? 128 : 64) void sweep_global_kernel(
     uint32_t *__restrict__ span_step, uint32_t *__restrict__ workspace,
     uint32_t rounds, uint64_t *__restrict__ chunk_state)
 {
+    // the handoff: root slots, the roots' codes, then (FOLDER) the records
     __shared__ uint32_t claims[CLAIM_SLOTS];
-    __shared__ uint32_t handoff[SPLIT ? SWEEP_HANDOFF_WORDS : 1u];
+    __shared__ uint32_t handoff[SPLIT ? SWEEP_HANDOFF_WORDS + (FOLDER ? 256u : 0u) : 1u];
     __shared__ uint32_t dirty[SPLIT ? DIRTY_SLOTS : 1u];
     const size_t cstride = (size_t)((M + 63u) / 64u) * PC_STATE_WORDS;
     const GlobalState st{workspace + (size_t)blockIdx.x * N};
     for (uint32_t r = blockIdx.x; r < R; r += gridDim.x) {
-        sweep_run<GlobalState, RecT, CHUNKS, SPLIT, ES>(
+        sweep_run<GlobalState, RecT, CHUNKS, SPLIT, ES, FOLDER>(
             st, N, M, edges, sides, span_first != 0, rounds, claims, handoff,
             dirty, perm + (int64_t)r * perm_stride, records + (int64_t)r * rec_stride,
             span_step + r, CHUNKS ? chunk_state + r * cstride : nullptr);
@@ -3879,7 +3885,9 @@ static int sweep_launch(uint32_t num_nodes, uint32_t num_edges, ES edges,
     hipStream_t s = as_stream(stream);
     const bool u32rec = num_nodes <= kU32RecordMaxNodes;
     const bool split = sweep_split(num_nodes, use_lds, waves);
-    const uint32_t threads = split ? 128u : 64u;
+    // HBM state with chunk states: a third wave folds the records
+    const bool folder = !use_lds && split && d_chunk_state;
+    const uint32_t threads = folder ? 192u : split ? 128u : 64u;
     if (use_lds) {
         const size_t lds = split ? sweep_lds_bytes<true>(num_nodes)
                                  : sweep_lds_bytes<false>(num_nodes);
@@ -3911,19 +3919,23 @@ static int sweep_launch(uint32_t num_nodes, uint32_t num_edges, ES edges,
         slots = num_runs;
     if (slots > kGlobalSlots)
         slots = kGlobalSlots;
-#define PC_SWEEP_GLOBAL(RT, CH, SP)                                            \
-    sweep_global_kernel<RT, CH, SP, ES><<<(uint32_t)slots, threads, 0, s>>>(       \
+#define PC_SWEEP_GLOBAL(RT, CH, SP, FO)                                        \
+    sweep_global_kernel<RT, CH, SP, ES, FO><<<(uint32_t)slots, threads, 0, s>>>(   \
         num_nodes, num_edges, num_runs, edges, d_node_sides,                   \
         span_on_first_merge, d_perm, perm_stride, (RT *)d_records, rec_stride, \
         d_span_step, (uint32_t *)d_workspace, rounds, d_chunk_state)
 #define PC_SWEEP_GLOBAL_W(RT, CH)                                              \
     do {                                                                       \
         if (split)                                                             \
-            PC_SWEEP_GLOBAL(RT, CH, true);                                     \
+            PC_SWEEP_GLOBAL(RT, CH, true, false);                              \
         else                                                                   \
-            PC_SWEEP_GLOBAL(RT, CH, false);                                    \
+            PC_SWEEP_GLOBAL(RT, CH, false, false);                             \
     } while (0)
-    if (u32rec && d_chunk_state)
+    if (folder && u32rec)
+        PC_SWEEP_GLOBAL(uint32_t, true, true, true);
+    else if (folder)
+        PC_SWEEP_GLOBAL(uint64_t, true, true, true);
+    else if (u32rec && d_chunk_state)
         PC_SWEEP_GLOBAL_W(uint32_t, true);
     else if (u32rec)
         PC_SWEEP_GLOBAL_W(uint32_t, false);
