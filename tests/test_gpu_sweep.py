@@ -360,3 +360,31 @@ def test_two_wavefronts_small_graphs(gpu, kind, length, placement):
     _check_sums(eng, res, expect)
     assert np.array_equal(res.chunk_state.cpu().numpy(),
                           eng.chunk_scan(res).cpu().numpy())
+
+
+@pytest.mark.parametrize('kind', ['star', 'complete', 'wheel'])
+def test_hbm_state_hub_graphs(gpu, kind):
+    """HBM state forced onto graphs whose merges keep hitting one root (a
+    star with extra rim edges, a complete graph, a wheel): the root the
+    finder hands over is changed by the resolver in nearly every batch, so
+    the resolver's stamp test and re-reads, the star merges and the folder
+    wave are all exercised.  Traces and sums equal the oracle's, and the
+    chunk states equal the separate pass."""
+    import networkx as nx
+    from pypercolate_amd import _native as nat
+    if kind == 'star':
+        gnx = nx.star_graph(3000)
+        gnx.add_edges_from((i, i + 1) for i in range(1, 3000, 7))
+    elif kind == 'complete':
+        gnx = nx.complete_graph(90)
+    else:
+        gnx = nx.wheel_graph(2500)
+    eng, g = _engine_for(gnx, False)
+    eng.state_mode = nat.PC_STATE_HBM
+    perms = oracle.randomstate_permutations(range(10), g['num_edges'])
+    expect = oracle.bond_runs(g, perms)
+    res = eng.sweep(eng.upload_permutations(perms), chunks=True)
+    _check_traces(eng, res, expect)
+    _check_sums(eng, res, expect)
+    assert np.array_equal(res.chunk_state.cpu().numpy(),
+                          eng.chunk_scan(res).cpu().numpy())
