@@ -158,8 +158,9 @@ int pc_permute_philox_workspace_size(uint32_t num_edges, uint32_t num_runs,
  * Wavefronts per run, OR-ed into state_mode of pc_sweep, pc_sweep_occupancy
  * and pc_sweep_workspace_size (neither: the library picks, two from 4096
  * nodes on or with the state in HBM).  Two: a finder wavefront finds the
- * roots of the next batch of bonds while the resolver merges this one.
- * Results do not depend on it.
+ * roots of the next batch of bonds while the resolver merges this one; with
+ * the state in HBM and d_chunk_state given, a third folds the records into
+ * the chunk states.  Results do not depend on it.
  */
 #define PC_SWEEP_ONE_WAVE 0x10
 #define PC_SWEEP_TWO_WAVES 0x20
