@@ -388,3 +388,22 @@ def test_hbm_state_hub_graphs(gpu, kind):
     _check_sums(eng, res, expect)
     assert np.array_equal(res.chunk_state.cpu().numpy(),
                           eng.chunk_scan(res).cpu().numpy())
+
+
+@pytest.mark.parametrize('chunks', [True, False])
+def test_hbm_sweep_with_fewer_slots_than_runs(gpu, chunks):
+    """A union-find workspace of 7 run slots for 40 runs: each workgroup
+    sweeps several runs in turn, clearing its slot, claim and stamp tables
+    between them.  Records, span steps and chunk states equal those of a
+    workspace with a slot per run."""
+    eng = BondEngine(_graph.square_lattice(130))
+    assert eng.N > 16384
+    perm = eng.device_permutations(11, 40)
+    full = eng.sweep(perm, chunks=chunks)
+    ws = gpu.empty((7 * eng.N * 4,), dtype=gpu.uint8, device='cuda')
+    part = eng.sweep(perm, workspace=ws, chunks=chunks)
+    assert np.array_equal(part.records.cpu().numpy(), full.records.cpu().numpy())
+    assert np.array_equal(part.span_step.cpu().numpy(), full.span_step.cpu().numpy())
+    if chunks:
+        assert np.array_equal(part.chunk_state.cpu().numpy(),
+                              full.chunk_state.cpu().numpy())
